@@ -1,0 +1,61 @@
+// internal.hpp -- host-side structures shared by the library's translation units.
+#pragma once
+
+#include "../../include/acceles.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+struct acc_coo {
+    int32_t rows = 0, cols = 0;
+    std::vector<int32_t> row, col;
+    std::vector<float> val;
+};
+
+struct acc_csr {
+    int32_t rows = 0, cols = 0;
+    std::vector<int64_t> ptr;  // rows + 1
+    std::vector<int32_t> idx;
+    std::vector<float> val;
+    int64_t nnz() const { return (int64_t)idx.size(); }
+};
+
+// Low-bit sliced packets.  Rows of each SPLIT partition are ordered by kept
+// entry count (descending, ties by row) and cut into 64-row slices, one row
+// per wavefront lane.  Lane l of slice s owns a bit stream of entries
+// (col: col_bits, value: bits), entry j at bits [j*E, (j+1)*E), E = entry_bits;
+// dword d of that stream lives at words[(slice_off[s] + d) * 64 + l].
+struct acc_packed {
+    int32_t rows = 0, cols = 0, split = 0;
+    int32_t bits = 8, col_bits = 10, entry_bits = 18, is_signed = 0;
+    float scale = 0.f;
+    int64_t nnz = 0, nnz_kept = 0;
+    std::vector<int32_t> part_row_begin;    // split + 1
+    std::vector<int64_t> part_slice_begin;  // split + 1
+    std::vector<uint32_t> slice_off;        // in units of 64 words
+    std::vector<uint32_t> slice_ndw;        // dwords per lane
+    std::vector<int32_t> perm;              // nslices * 64, -1 = padding lane
+    std::vector<uint32_t> words;
+    int64_t nslices() const { return (int64_t)slice_off.size(); }
+};
+
+namespace acc {
+
+int fail(int code, const std::string &msg);
+void clear_error();
+
+// ranking keys: larger key ranks first; score descending then row ascending
+inline uint64_t key_i32(int32_t score, int32_t row) {
+    return ((uint64_t)((uint32_t)score ^ 0x80000000u) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)row);
+}
+inline uint32_t ordered_f32(float f) {
+    if (f == 0.0f) f = 0.0f;  // -0 == +0 ranks as a tie
+    uint32_t b;
+    __builtin_memcpy(&b, &f, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+int bits_for(int32_t n);  // ceil(log2(n)), at least 1
+
+}  // namespace acc

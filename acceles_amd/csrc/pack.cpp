@@ -1,0 +1,283 @@
+// pack.cpp -- low-bit packing.
+//
+//   * Sliced packets (the GPU layout): quantize with one global FP32 scale
+//     (quantify_util.py:8-28), drop entries that quantize to zero
+//     (csr_util.py:13-22), split rows into SPLIT partitions
+//     (quantify_util.py:184-189), order each partition's rows by entry count
+//     and cut them into 64-row slices, one row per wavefront lane, each lane
+//     a continuous bit stream of (column, value) entries with the dwords of
+//     the 64 lanes interleaved so a wavefront's dword load is one 256-byte
+//     contiguous segment.
+//   * Ultra-CSR packets (the reference's FPGA format,
+//     host_spmv_ucsr.cpp:388-425, fpga_utils.hpp:438-503), with the
+//     x-decoder (fpga_utils.hpp:279-302) for reading them back.
+#include "internal.hpp"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cstring>
+
+using acc::fail;
+
+// ---------------------------------------------------------------------------
+// sliced packets
+// ---------------------------------------------------------------------------
+
+extern "C" int acc_pack_lowbit(const acc_csr *m, const acc_pack_opts *o, acc_packed **out) {
+    if (!m || !o || !out) return fail(ACC_ERR_ARG, "null argument");
+    if (o->bits < 2 || o->bits > 8) return fail(ACC_ERR_ARG, "value bits must be in [2, 8]");
+    if (o->split < 1 || o->split > m->rows) return fail(ACC_ERR_ARG, "split must be in [1, rows]");
+    int cb = o->col_bits > 0 ? o->col_bits : acc::bits_for(m->cols);
+    if ((int64_t(1) << cb) < m->cols) return fail(ACC_ERR_ARG, "col_bits too small for the column count");
+    int E = cb + o->bits;
+    if (E < 10) { cb += 10 - E; E = 10; }  // narrowest decoder instantiated on the GPU
+    if (E > 22) return fail(ACC_ERR_ARG, "column bits + value bits must be <= 22 (cols <= 16384 at 8 bits)");
+    const int64_t nnz = m->nnz();
+
+    acc_packed *pk = new acc_packed();
+    pk->rows = m->rows;
+    pk->cols = m->cols;
+    pk->split = o->split;
+    pk->bits = o->bits;
+    pk->col_bits = cb;
+    pk->entry_bits = E;
+    pk->is_signed = o->is_signed ? 1 : 0;
+    pk->nnz = nnz;
+
+    if (!pk->is_signed) {
+        int neg = 0;
+#pragma omp parallel for reduction(| : neg)
+        for (int64_t i = 0; i < nnz; ++i)
+            if (m->val[(size_t)i] < 0.f) neg = 1;
+        if (neg) { delete pk; return fail(ACC_ERR_ARG, "unsigned packing needs non-negative values (use is_signed=1)"); }
+    }
+    std::vector<int32_t> q((size_t)nnz);
+    int rc = acc_quantize(nnz, m->val.data(), o->bits, pk->is_signed, &pk->scale, q.data());
+    if (rc != ACC_OK) { delete pk; return rc; }
+
+    const int32_t R = m->rows;
+    std::vector<int32_t> kept((size_t)R);
+    int64_t nkept = 0;
+#pragma omp parallel for reduction(+ : nkept) schedule(static, 4096)
+    for (int32_t r = 0; r < R; ++r) {
+        int32_t c = 0;
+        for (int64_t j = m->ptr[(size_t)r]; j < m->ptr[(size_t)r + 1]; ++j) c += q[(size_t)j] != 0;
+        kept[(size_t)r] = c;
+        nkept += c;
+    }
+    pk->nnz_kept = nkept;
+
+    // partitions (quantify_util.py:185-189): rows/split each, remainder to the last
+    const int S = o->split;
+    const int32_t psize = R / S;
+    pk->part_row_begin.resize((size_t)S + 1);
+    for (int p = 0; p < S; ++p) pk->part_row_begin[(size_t)p] = p * psize;
+    pk->part_row_begin[(size_t)S] = R;
+    pk->part_slice_begin.resize((size_t)S + 1);
+    pk->part_slice_begin[0] = 0;
+    for (int p = 0; p < S; ++p) {
+        int32_t n = pk->part_row_begin[(size_t)p + 1] - pk->part_row_begin[(size_t)p];
+        pk->part_slice_begin[(size_t)p + 1] = pk->part_slice_begin[(size_t)p] + (n + 63) / 64;
+    }
+    const int64_t NS = pk->part_slice_begin[(size_t)S];
+    pk->perm.assign((size_t)NS * 64, -1);
+    pk->slice_ndw.assign((size_t)NS, 0);
+    pk->slice_off.assign((size_t)NS, 0);
+
+    // order rows of each partition by kept count descending, row ascending
+    // (counting sort: stable, O(rows))
+#pragma omp parallel for schedule(dynamic)
+    for (int p = 0; p < S; ++p) {
+        int32_t b = pk->part_row_begin[(size_t)p], e = pk->part_row_begin[(size_t)p + 1];
+        int32_t maxc = 0;
+        for (int32_t r = b; r < e; ++r) maxc = std::max(maxc, kept[(size_t)r]);
+        std::vector<int64_t> start((size_t)maxc + 2, 0);
+        for (int32_t r = b; r < e; ++r) start[(size_t)(maxc - kept[(size_t)r]) + 1]++;
+        for (int32_t c = 0; c <= maxc; ++c) start[(size_t)c + 1] += start[(size_t)c];
+        int32_t *dst = pk->perm.data() + pk->part_slice_begin[(size_t)p] * 64;
+        for (int32_t r = b; r < e; ++r) dst[start[(size_t)(maxc - kept[(size_t)r])]++] = r;
+        for (int64_t s = pk->part_slice_begin[(size_t)p]; s < pk->part_slice_begin[(size_t)p + 1]; ++s) {
+            int32_t r0 = pk->perm[(size_t)s * 64];  // longest row of the slice
+            int64_t cnt = r0 >= 0 ? kept[(size_t)r0] : 0;
+            pk->slice_ndw[(size_t)s] = (uint32_t)((cnt * E + 31) / 32);
+        }
+    }
+    uint64_t off = 0;
+    for (int64_t s = 0; s < NS; ++s) {
+        if (off > 0xFFFFFFFFull) { delete pk; return fail(ACC_ERR_RANGE, "packed matrix exceeds 2^38 bytes"); }
+        pk->slice_off[(size_t)s] = (uint32_t)off;
+        off += pk->slice_ndw[(size_t)s];
+    }
+    pk->words.assign((size_t)off * 64, 0u);
+
+    const uint32_t vmask = (1u << o->bits) - 1u;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t s = 0; s < NS; ++s) {
+        uint32_t *base = pk->words.data() + (size_t)pk->slice_off[(size_t)s] * 64;
+        for (int l = 0; l < 64; ++l) {
+            int32_t r = pk->perm[(size_t)s * 64 + l];
+            if (r < 0) continue;
+            int64_t j = 0;
+            for (int64_t t = m->ptr[(size_t)r]; t < m->ptr[(size_t)r + 1]; ++t) {
+                int32_t qv = q[(size_t)t];
+                if (qv == 0) continue;
+                uint32_t x = (uint32_t)m->idx[(size_t)t] | (((uint32_t)qv & vmask) << cb);
+                int64_t pos = j * E;
+                int64_t w = pos >> 5;
+                int sh = (int)(pos & 31);
+                base[(size_t)w * 64 + l] |= x << sh;
+                if (sh + E > 32) base[(size_t)(w + 1) * 64 + l] |= x >> (32 - sh);
+                ++j;
+            }
+        }
+    }
+    *out = pk;
+    return ACC_OK;
+}
+
+extern "C" int acc_packed_info(const acc_packed *pk, acc_packed_info_t *info) {
+    if (!pk || !info) return fail(ACC_ERR_ARG, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->rows = pk->rows;
+    info->cols = pk->cols;
+    info->split = pk->split;
+    info->bits = pk->bits;
+    info->col_bits = pk->col_bits;
+    info->entry_bits = pk->entry_bits;
+    info->is_signed = pk->is_signed;
+    info->nnz = pk->nnz;
+    info->nnz_kept = pk->nnz_kept;
+    info->nslices = pk->nslices();
+    info->stream_bytes = (int64_t)pk->words.size() * 4 + pk->nslices() * 8;
+    info->perm_bytes = (int64_t)pk->perm.size() * 4;
+    info->bytes_per_nnz = pk->nnz > 0 ? (double)info->stream_bytes / (double)pk->nnz : 0.0;
+    info->scale = pk->scale;
+    return ACC_OK;
+}
+
+extern "C" int acc_unpack_lowbit(const acc_packed *pk, int64_t *ptr, int32_t *idx, int32_t *qval) {
+    if (!pk || !ptr) return fail(ACC_ERR_ARG, "null argument");
+    const int E = pk->entry_bits, cb = pk->col_bits, vb = pk->bits;
+    const uint32_t cmask = (1u << cb) - 1u, vmask = (1u << vb) - 1u;
+    // per row: its slot (slice, lane)
+    std::vector<int64_t> slot_of((size_t)pk->rows, -1);
+    for (int64_t i = 0; i < (int64_t)pk->perm.size(); ++i)
+        if (pk->perm[(size_t)i] >= 0) slot_of[(size_t)pk->perm[(size_t)i]] = i;
+    // decode every lane stream; entries end at the first zero value
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> rows((size_t)pk->rows);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int32_t r = 0; r < pk->rows; ++r) {
+        int64_t sl = slot_of[(size_t)r];
+        if (sl < 0) continue;
+        int64_t s = sl / 64;
+        int l = (int)(sl % 64);
+        const uint32_t *base = pk->words.data() + (size_t)pk->slice_off[(size_t)s] * 64;
+        int64_t nent = (int64_t)pk->slice_ndw[(size_t)s] * 32 / E;
+        for (int64_t j = 0; j < nent; ++j) {
+            int64_t pos = j * E, w = pos >> 5;
+            int sh = (int)(pos & 31);
+            uint64_t lo = base[(size_t)w * 64 + l];
+            uint64_t hi = (w + 1 < (int64_t)pk->slice_ndw[(size_t)s]) ? base[(size_t)(w + 1) * 64 + l] : 0;
+            uint32_t x = (uint32_t)(((hi << 32) | lo) >> sh);
+            uint32_t v = (x >> cb) & vmask;
+            if (v == 0) break;
+            int32_t sv = (int32_t)v;
+            if (pk->is_signed && (v >> (vb - 1))) sv = (int32_t)(v | ~vmask);
+            rows[(size_t)r].push_back({(int32_t)(x & cmask), sv});
+        }
+    }
+    ptr[0] = 0;
+    for (int32_t r = 0; r < pk->rows; ++r) ptr[(size_t)r + 1] = ptr[(size_t)r] + (int64_t)rows[(size_t)r].size();
+    if (idx || qval) {
+        for (int32_t r = 0; r < pk->rows; ++r) {
+            int64_t d = ptr[(size_t)r];
+            for (auto &e : rows[(size_t)r]) {
+                if (idx) idx[d] = e.first;
+                if (qval) qval[d] = e.second;
+                ++d;
+            }
+        }
+    }
+    return ACC_OK;
+}
+
+extern "C" void acc_packed_destroy(acc_packed *pk) { delete pk; }
+
+// ---------------------------------------------------------------------------
+// Ultra-CSR reference packets
+// ---------------------------------------------------------------------------
+
+extern "C" int acc_ultracsr_entries(int col_bits, int val_bits) {
+    if (col_bits < 1 || val_bits < 1 || 1 + col_bits + val_bits > 511)
+        return fail(ACC_ERR_ARG, "bad Ultra-CSR field widths");
+    return 511 / (1 + col_bits + val_bits);
+}
+
+namespace {
+inline void set_field(uint64_t *w, int pos, int len, uint64_t v) {
+    v &= (len >= 64) ? ~0ull : ((1ull << len) - 1);
+    int i = pos >> 6, sh = pos & 63;
+    w[i] |= v << sh;
+    if (sh + len > 64) w[i + 1] |= v >> (64 - sh);
+}
+inline uint64_t get_field(const uint64_t *w, int pos, int len) {
+    int i = pos >> 6, sh = pos & 63;
+    uint64_t v = w[i] >> sh;
+    if (sh + len > 64) v |= w[i + 1] << (64 - sh);
+    return v & ((len >= 64) ? ~0ull : ((1ull << len) - 1));
+}
+}  // namespace
+
+extern "C" int64_t acc_pack_ultracsr(int64_t nnz, const int32_t *row, const int32_t *col, const int32_t *qval,
+                                     int32_t last_row, int col_bits, int val_bits, uint64_t *packets, int64_t cap) {
+    int n = acc_ultracsr_entries(col_bits, val_bits);
+    if (n < 0) return n;
+    if (nnz < 0 || (nnz > 0 && (!row || !col || !qval || !packets))) return fail(ACC_ERR_ARG, "bad arguments");
+    int64_t np = (nnz + n - 1) / n;
+    if (np > cap) return fail(ACC_ERR_ARG, "packet buffer too small");
+    std::memset(packets, 0, sizeof(uint64_t) * 8 * (size_t)np);
+    int32_t prev = last_row;
+    for (int64_t i = 0; i < nnz; ++i) {
+        uint64_t *w = packets + 8 * (i / n);
+        int j = (int)(i % n);
+        if (col[i] < 0 || (col_bits < 31 && col[i] >= (1 << col_bits)))
+            return fail(ACC_ERR_RANGE, "column does not fit the column field");
+        set_field(w, j, 1, row[i] != prev ? 1u : 0u);
+        prev = row[i];
+        set_field(w, n + j * col_bits, col_bits, (uint32_t)col[i]);
+        set_field(w, n + n * col_bits + j * val_bits, val_bits, (uint32_t)qval[i]);
+    }
+    return np;
+}
+
+extern "C" int acc_xdecode(uint32_t flags, int n, int32_t carry, int32_t *x) {
+    if (!x || n < 1 || n > 32) return fail(ACC_ERR_ARG, "n must be in [1, 32]");
+    for (int i = 0; i < n; ++i) {
+        uint32_t mask = i >= 31 ? 0xFFFFFFFFu : ((2u << i) - 1u);
+        x[i] = carry + __builtin_popcount(flags & mask);
+    }
+    return ACC_OK;
+}
+
+extern "C" int acc_unpack_ultracsr(const uint64_t *packets, int64_t nnz, int col_bits, int val_bits, int32_t carry0,
+                                   int32_t *xrow, int32_t *col, int32_t *qval) {
+    int n = acc_ultracsr_entries(col_bits, val_bits);
+    if (n < 0) return n;
+    if (n > 32) return fail(ACC_ERR_ARG, "more than 32 entries per packet: flags exceed the decoder width");
+    int32_t carry = carry0, x[32];
+    for (int64_t p = 0; p * n < nnz; ++p) {
+        const uint64_t *w = packets + 8 * p;
+        acc_xdecode((uint32_t)get_field(w, 0, n), n, carry, x);
+        carry = x[n - 1];
+        for (int j = 0; j < n && p * n + j < nnz; ++j) {
+            int64_t i = p * n + j;
+            if (xrow) xrow[i] = x[j];
+            if (col) col[i] = (int32_t)get_field(w, n + j * col_bits, col_bits);
+            uint32_t v = (uint32_t)get_field(w, n + n * col_bits + j * val_bits, val_bits);
+            if (v >> (val_bits - 1)) v |= ~((1u << val_bits) - 1u);
+            if (qval) qval[i] = (int32_t)v;
+        }
+    }
+    return ACC_OK;
+}

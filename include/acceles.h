@@ -1,0 +1,273 @@
+/*
+ * acceles.h -- C ABI of the MI355X-native AccelES Top-K SpMV library
+ * (libacceles.so).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every function returns an int status: ACC_OK (0) or a negative
+ *     ACC_ERR_* code; acc_last_error() gives the message of the calling
+ *     thread's last failure.  The library never exits the process (the
+ *     reference exits or prints and continues, utils.hpp:460-472,512-513).
+ *   - Handles (acc_coo, acc_csr, acc_packed, acc_gpu_matrix) are owned by the
+ *     library: create/destroy pairs.  Output arrays are owned by the caller.
+ *   - A handle is not shared between threads without external locking.
+ *     GPU calls enqueue on the caller's stream (hipStream_t passed as void*,
+ *     NULL = the null stream) and return without synchronising, except the
+ *     *_host convenience calls, which copy in, run and copy out.
+ *   - Ranking order everywhere: score descending, then row index ascending.
+ *
+ * Each declaration cites the reference interface it replaces (path:line in
+ * the AccelES tree).
+ */
+#ifndef ACCELES_H
+#define ACCELES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACC_OK 0
+#define ACC_ERR_ARG (-1)      /* bad argument                         */
+#define ACC_ERR_IO (-2)       /* file cannot be opened / read         */
+#define ACC_ERR_FORMAT (-3)   /* malformed Matrix Market content      */
+#define ACC_ERR_RANGE (-4)    /* index or size out of range           */
+#define ACC_ERR_NOMEM (-5)    /* host or device allocation failed     */
+#define ACC_ERR_DEVICE (-6)   /* HIP runtime error                    */
+#define ACC_ERR_NODEVICE (-7) /* no GPU visible                       */
+#define ACC_ERR_BUDGET (-8)   /* serving admission: memory budget     */
+
+/* One ranked result; same layout as Element (src/cpu/cpu_mkl_baseline.cpp:16-19). */
+typedef struct {
+    float value;
+    int32_t index;
+} acc_element;
+
+typedef struct acc_coo acc_coo;               /* host COO triples            */
+typedef struct acc_csr acc_csr;               /* host FP32 CSR               */
+typedef struct acc_packed acc_packed;         /* host low-bit sliced packets */
+typedef struct acc_gpu_matrix acc_gpu_matrix; /* device-resident matrix      */
+
+const char *acc_last_error(void);
+const char *acc_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Loaders                                                             */
+/* ------------------------------------------------------------------ */
+
+/* read_coo_matrix (src/cpu/cpu_mkl_baseline.cpp:43-77): banner line, one
+ * more header line, then "rows cols nnz" and nnz "r c v" lines; indices are
+ * 1-based unless zero_index != 0.  Parsed in parallel. */
+int acc_read_coo_matrix(const char *path, int zero_index, acc_coo **out);
+
+/* readMtx (src/common/utils/utils.hpp:449-492): Matrix Market coordinate
+ * file, any number of '%' comment lines; 1-based unless zero_indexed_file;
+ * pattern files read values as 1.0; sort_tuples orders entries by (row, col)
+ * (stable). */
+int acc_read_mtx(const char *path, int zero_indexed_file, int sort_tuples, acc_coo **out);
+
+/* write_coo_to_mtx (src/resources/python/quantify/quantify_util.py:469-477):
+ * "%%MatrixMarket matrix coordinate real general", "%", size line, 1-based
+ * triples with values printed to 9 significant digits. */
+int acc_write_mtx(const char *path, const acc_coo *coo);
+
+/* coo_t (src/fpga/src/ip/coo_matrix.hpp:9-24) / COOMatrix
+ * (cpu_mkl_baseline.cpp:33-40) built from caller arrays (copied). */
+int acc_coo_create(int32_t rows, int32_t cols, int64_t nnz, const int32_t *row,
+                   const int32_t *col, const float *val, acc_coo **out);
+int acc_coo_info(const acc_coo *coo, int32_t *rows, int32_t *cols, int64_t *nnz);
+int acc_coo_export(const acc_coo *coo, int32_t *row, int32_t *col, float *val);
+void acc_coo_destroy(acc_coo *coo);
+
+/* ------------------------------------------------------------------ */
+/* Conversion                                                          */
+/* ------------------------------------------------------------------ */
+
+/* coo2csr (src/common/utils/utils.hpp:494-547), caller-allocated outputs:
+ * ptr[nrows+1], idx[nnz], val_out[nnz].  Counting sort, stable within a row;
+ * sort_tuples first orders by (row, col).  Out-of-range indices return
+ * ACC_ERR_RANGE instead of printing and continuing. */
+int acc_coo2csr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *row,
+                const int32_t *col, const float *val, int sort_tuples, int64_t *ptr,
+                int32_t *idx, float *val_out);
+
+/* coo_to_csr (src/cpu/cpu_mkl_baseline.cpp:79-109): library-owned CSR handle. */
+int acc_coo_to_csr(const acc_coo *coo, acc_csr **out);
+int acc_csr_create(int32_t rows, int32_t cols, int64_t nnz, const int64_t *ptr,
+                   const int32_t *idx, const float *val, acc_csr **out);
+int acc_csr_info(const acc_csr *csr, int32_t *rows, int32_t *cols, int64_t *nnz);
+int acc_csr_export(const acc_csr *csr, int64_t *ptr, int32_t *idx, float *val);
+void acc_csr_destroy(acc_csr *csr);
+
+/* quantify_matrix / quantify_vector (quantify_util.py:8-49): M = 2^bits-1
+ * (halved when symmetric), s = max|v|, q = clip(rint(M*(v/s)), -M-1, M) in
+ * float32 with round-half-even; s is written to *scale_out. */
+int acc_quantize(int64_t n, const float *v, int bits, int symmetric, float *scale_out,
+                 int32_t *q);
+
+/* Random-CSR (README.md:5; row shuffle of quantify_util.py:491 and
+ * csr_util.py:101-114), seeded: out row i is input row perm[i]. */
+int acc_permute_rows(const acc_csr *csr, uint64_t seed, acc_csr **out, int32_t *perm);
+
+/* ---- Ultra-CSR reference packet format ---------------------------- */
+
+/* Entries per 512-bit packet: 511 / (1 + col_bits + val_bits)
+ * (src/common/types.hpp:151-162; 30 for 10-bit columns, 6-bit values). */
+int acc_ultracsr_entries(int col_bits, int val_bits);
+
+/* Q_packet_coo_partition (src/fpga/src/host_spmv_ucsr.cpp:388-425) with
+ * Q_write_block_x/y/val (src/fpga/src/ip/fpga_utils.hpp:438-503): one
+ * partition's COO stream -> packets of 8 little-endian 64-bit words.  Flag =
+ * (row != previous entry's row), the previous row starting at last_row.
+ * Returns the packet count (>= 0) or an error; cap is in packets. */
+int64_t acc_pack_ultracsr(int64_t nnz, const int32_t *row, const int32_t *col,
+                          const int32_t *qval, int32_t last_row, int col_bits, int val_bits,
+                          uint64_t *packets, int64_t cap);
+
+/* Q_read_block_x/y/val (fpga_utils.hpp:317-384): per entry the x-decoded row
+ * counter (carried across packets from carry0), column and value. */
+int acc_unpack_ultracsr(const uint64_t *packets, int64_t nnz, int col_bits, int val_bits,
+                        int32_t carry0, int32_t *xrow, int32_t *col, int32_t *qval);
+
+/* increment_array / adder (fpga_utils.hpp:279-302, x_decoder.cpp:5-22):
+ * x[i] = carry + popcount(flags & ((2 << i) - 1)), i < n <= 32. */
+int acc_xdecode(uint32_t flags, int n, int32_t carry, int32_t *x);
+
+/* ---- Low-bit sliced packets (the GPU layout) ----------------------- */
+
+typedef struct {
+    int bits;      /* value bits B, 2..8 (4 and 8 are the tuned cases)       */
+    int is_signed; /* 0: unsigned [0, 2^B-1] (non-negative input required),
+                      1: symmetric signed [-2^(B-1), 2^(B-1)-1]              */
+    int split;     /* SPLIT row partitions (quantify_util.py:184-189)        */
+    int col_bits;  /* 0 = ceil(log2(cols))                                   */
+} acc_pack_opts;
+
+typedef struct {
+    int32_t rows, cols, split, bits, col_bits, entry_bits, is_signed;
+    int64_t nnz;          /* stored entries of the source CSR                */
+    int64_t nnz_kept;     /* entries whose quantized value is non-zero       */
+    int64_t nslices;      /* 64-row slices                                   */
+    int64_t stream_bytes; /* packet words + per-slice metadata (read per query) */
+    int64_t perm_bytes;   /* slot -> row map (read only for candidates)      */
+    double bytes_per_nnz; /* stream_bytes / nnz                               */
+    float scale;          /* global FP32 scale of the values                 */
+} acc_packed_info_t;
+
+/* Quantize (acc_quantize rules) and pack the CSR (host_spmv_ucsr.cpp:278-319
+ * partitions + packing, re-laid-out for wavefronts). */
+int acc_pack_lowbit(const acc_csr *csr, const acc_pack_opts *opts, acc_packed **out);
+int acc_packed_info(const acc_packed *pk, acc_packed_info_t *info);
+/* Decode back to CSR order: ptr[rows+1], idx[nnz_kept], qval[nnz_kept]. */
+int acc_unpack_lowbit(const acc_packed *pk, int64_t *ptr, int32_t *idx, int32_t *qval);
+void acc_packed_destroy(acc_packed *pk);
+
+/* ---- Synthetic inputs (create_matrices.py:57-137, seeded) ---------- */
+
+#define ACC_DEG_UNIFORM 0 /* row nnz uniform in [p0, p1]                      */
+#define ACC_DEG_GAMMA 1   /* row nnz = clamp(round(Gamma(p0, p1)), 1, cols)   */
+#define ACC_VAL_ABSNORMAL 0
+#define ACC_VAL_NORMAL 1
+#define ACC_VAL_UNIFORM 2 /* U(-1, 1)                                         */
+
+/* Distinct uniform columns per row (sorted), values per val_kind,
+ * optional L2 row normalisation.  Deterministic for a seed. */
+int acc_generate_csr(int32_t rows, int32_t cols, int deg_kind, double p0, double p1,
+                     int val_kind, int l2norm, uint64_t seed, acc_csr **out);
+int acc_generate_queries(int32_t nq, int32_t cols, int val_kind, int l2norm, uint64_t seed,
+                         float *out);
+
+/* ------------------------------------------------------------------ */
+/* GPU (HIP, gfx950)                                                   */
+/* ------------------------------------------------------------------ */
+
+int acc_gpu_device_count(int *count);
+
+/* SpMV::setup (src/gpu/host_spmv_topk_csr_gpu.cu:104-169) and Q_setup
+ * (host_spmv_ucsr.cpp:501-539): copy the FP32 CSR and (optionally) the
+ * low-bit packets to device memory on `device`.  Either may be NULL but not
+ * both; the two-stage path needs both (stage 2 rescoring reads the CSR). */
+int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int device, acc_gpu_matrix **out);
+
+typedef struct {
+    int32_t rows, cols, split, bits, has_csr, has_packed;
+    int32_t group;         /* G: lanes per row in the FP32 dot products      */
+    int32_t work_items;    /* stage-1 wavefront work items                   */
+    int64_t nnz;
+    int64_t csr_bytes;     /* FP32 CSR bytes read per query                  */
+    int64_t stream_bytes;  /* low-bit bytes read per stage-1 pass            */
+    int64_t device_bytes;  /* total device memory held by the handle         */
+} acc_gpu_info_t;
+
+int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info);
+void acc_gpu_matrix_destroy(acc_gpu_matrix *m);
+
+/* Plain FP32 CSR Top-K: SpMV (cpu_mkl_baseline.cpp:261) fused with the Top-k
+ * of top_k_heap (:128-149); the GPU baseline spec is
+ * host_spmv_topk_csr_gpu.cu:171-231.  k <= 1024.  x_dev: cols floats;
+ * out_dev: k elements in ranking order (tail (-inf, -1) if k > rows). */
+int acc_gpu_topk_fp32(acc_gpu_matrix *m, const float *x_dev, int k, acc_element *out_dev,
+                      void *stream);
+
+/* Low-bit two-stage Top-K: query quantized to signed 8-bit with its own scale
+ * (quantify_util.py:259), INT32 scores of the packed matrix, top floor(ck/split)
+ * rows per partition (quantify_util.py:173-196 / spmv_ucsr_top_k_multicore.hpp
+ * :360-413), merge (host_spmv_ucsr.cpp:691-748), FP32 rescoring of the
+ * candidates (quantify_util.py:354-367) and the final Top-k.
+ * k <= ck, ck/split <= 64. */
+int acc_gpu_topk_lowbit(acc_gpu_matrix *m, const float *x_dev, int k, int ck,
+                        acc_element *out_dev, void *stream);
+
+/* The two launches of acc_gpu_topk_lowbit, separately: stage 1 (INT32 SpMV
+ * and per-partition select into the handle's candidate lists; the FPGA
+ * kernel launch of host_spmv_ucsr.cpp:556-565) and stage 2 (merge, FP32
+ * rescoring, Top-K; Q_read_result :691-748 plus quantify_util.py:354-367).
+ * Stage 2 must follow stage 1 on the same stream with the same x and ck. */
+int acc_gpu_lowbit_stage1(acc_gpu_matrix *m, const float *x_dev, int ck, void *stream);
+int acc_gpu_lowbit_stage2(acc_gpu_matrix *m, const float *x_dev, int k, int ck, acc_element *out_dev,
+                          void *stream);
+
+/* FP64 exact scores y_dev[rows] = A x with FP64 accumulation (the exact
+ * reference of quantify_util.py:257-258), for precision/recall reporting. */
+int acc_gpu_spmv_fp64(acc_gpu_matrix *m, const float *x_dev, double *y_dev, void *stream);
+
+/* Batched low-bit two-stage: nq queries (row-major nq x cols), query i keeps
+ * k_host[i] results in out_dev[i*out_stride ...].  One matrix pass scores a
+ * block of queries. */
+int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq,
+                              const int32_t *k_host, int ck, acc_element *out_dev,
+                              int out_stride, void *stream);
+
+/* Stage outputs, for parity checks.
+ *   spmv_fp32:      y_dev[rows] FP32 scores (the Top-K path's order)
+ *   scores_lowbit:  y_dev[rows] INT32 stage-1 scores, original row order
+ *   candidates:     per partition its top floor(ck/split) (row, INT32 score)
+ *                   in ranking order, partitions in order; *count_host gets
+ *                   the total (synchronises the stream). */
+int acc_gpu_spmv_fp32(acc_gpu_matrix *m, const float *x_dev, float *y_dev, void *stream);
+int acc_gpu_scores_lowbit(acc_gpu_matrix *m, const float *x_dev, int32_t *y_dev, void *stream);
+int acc_gpu_candidates_lowbit(acc_gpu_matrix *m, const float *x_dev, int ck, int32_t *rows_dev,
+                              int32_t *scores_dev, int *count_host, void *stream);
+
+/* top_k_heap (cpu_mkl_baseline.cpp:128-149) over a device score vector. */
+int acc_gpu_top_k_heap(const float *y_dev, int n, int k, acc_element *out_dev, void *stream);
+
+/* Host-pointer convenience forms: copy in, run on the handle's device,
+ * copy out, synchronise. */
+int acc_topk_fp32_host(acc_gpu_matrix *m, const float *x_host, int k, acc_element *out_host);
+int acc_topk_lowbit_host(acc_gpu_matrix *m, const float *x_host, int k, int ck,
+                         acc_element *out_host);
+
+/* top_k_spmv (cpu_mkl_baseline.cpp:249-275) drop-in: num_test queries of
+ * U(-1,1) values (generate_random_vector :111-122, seeded here), each one
+ * FP32 SpMV + Top-k on the GPU; prints the reference's three timing lines
+ * and leaves the last query's results in top_k.  num_threads is accepted for
+ * signature compatibility and ignored. */
+int acc_top_k_spmv(acc_gpu_matrix *A, int row, int col, int k, int num_threads,
+                   acc_element *top_k, int num_test);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ACCELES_H */
