@@ -102,6 +102,17 @@ SIGNATURES = [
                                    C.c_uint64, C.POINTER(_P)]),
     ("acc_generate_queries", C.c_int, [C.c_int32, C.c_int32, C.c_int, C.c_int, C.c_uint64, _F32P]),
     ("acc_gpu_device_count", C.c_int, [C.POINTER(C.c_int)]),
+    ("acc_dev_malloc", C.c_int, [C.c_int, C.c_size_t, C.POINTER(_P)]),
+    ("acc_dev_free", C.c_int, [_P]),
+    ("acc_memcpy_h2d", C.c_int, [_P, _P, C.c_size_t]),
+    ("acc_memcpy_d2h", C.c_int, [_P, _P, C.c_size_t]),
+    ("acc_stream_create", C.c_int, [C.c_int, C.POINTER(_P)]),
+    ("acc_stream_destroy", C.c_int, [_P]),
+    ("acc_stream_synchronize", C.c_int, [_P]),
+    ("acc_event_create", C.c_int, [C.POINTER(_P)]),
+    ("acc_event_record", C.c_int, [_P, _P]),
+    ("acc_event_elapsed_ms", C.c_int, [_P, _P, C.POINTER(C.c_float)]),
+    ("acc_event_destroy", C.c_int, [_P]),
     ("acc_gpu_upload", C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
     ("acc_gpu_matrix_info", C.c_int, [_P, C.POINTER(GpuInfo)]),
     ("acc_gpu_matrix_destroy", None, [_P]),
@@ -365,14 +376,98 @@ def device_count() -> int:
     return n.value
 
 
+class DeviceBuffer:
+    """Device memory owned by this object (acc_dev_malloc); typed like a numpy array."""
+
+    def __init__(self, shape, dtype, device=0):
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.device = device
+        p = _P()
+        _check(lib().acc_dev_malloc(device, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a, device=0):
+        a = np.ascontiguousarray(a)
+        b = cls(a.shape, a.dtype, device)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes <= self.nbytes
+        _check(lib().acc_memcpy_h2d(self.ptr, a.ctypes.data_as(_P), a.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().acc_memcpy_d2h(out.ctypes.data_as(_P), self.ptr, self.nbytes))
+        return out
+
+    def at(self, offset_elems):
+        """Raw device address offset_elems elements into the buffer."""
+        return self.ptr.value + int(offset_elems) * self.dtype.itemsize
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value and _lib is not None:
+            _lib.acc_dev_free(self.ptr)
+            self.ptr = None
+
+
+class Stream:
+    def __init__(self, device=0):
+        s = _P()
+        _check(lib().acc_stream_create(device, C.byref(s)))
+        self.handle = s
+
+    def synchronize(self):
+        _check(lib().acc_stream_synchronize(self.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and self.handle.value and _lib is not None:
+            _lib.acc_stream_destroy(self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        e = _P()
+        _check(lib().acc_event_create(C.byref(e)))
+        self.handle = e
+
+    def record(self, stream=None):
+        _check(lib().acc_event_record(self.handle, _stream(stream)))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        _check(lib().acc_event_elapsed_ms(self.handle, end.handle, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and self.handle.value and _lib is not None:
+            _lib.acc_event_destroy(self.handle)
+            self.handle = None
+
+
+def synchronize(stream=None):
+    _check(lib().acc_stream_synchronize(_stream(stream)))
+
+
 def _ptr(t):
-    """Device pointer of a torch tensor (or an int)."""
-    return _P(t if isinstance(t, int) else t.data_ptr())
+    """Device address of a DeviceBuffer, a raw int, or an object with data_ptr()."""
+    if isinstance(t, DeviceBuffer):
+        return t.ptr
+    if isinstance(t, int):
+        return _P(t)
+    return _P(t.data_ptr())
 
 
 def _stream(stream):
     if stream is None:
         return _P()
+    if isinstance(stream, Stream):
+        return stream.handle
     return _P(stream if isinstance(stream, int) else stream.cuda_stream)
 
 

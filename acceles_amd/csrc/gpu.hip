@@ -261,6 +261,69 @@ extern "C" int acc_gpu_device_count(int *count) {
     return ACC_OK;
 }
 
+extern "C" int acc_dev_malloc(int device, size_t bytes, void **ptr) {
+    if (!ptr) return fail(ACC_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(device));
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(ACC_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return ACC_OK;
+}
+
+extern "C" int acc_dev_free(void *ptr) {
+    HIP_TRY(hipFree(ptr));
+    return ACC_OK;
+}
+
+extern "C" int acc_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return ACC_OK;
+}
+
+extern "C" int acc_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return ACC_OK;
+}
+
+extern "C" int acc_stream_create(int device, void **stream) {
+    if (!stream) return fail(ACC_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags((hipStream_t *)stream, hipStreamNonBlocking));
+    return ACC_OK;
+}
+
+extern "C" int acc_stream_destroy(void *stream) {
+    HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return ACC_OK;
+}
+
+extern "C" int acc_stream_synchronize(void *stream) {
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return ACC_OK;
+}
+
+extern "C" int acc_event_create(void **event) {
+    if (!event) return fail(ACC_ERR_ARG, "null argument");
+    HIP_TRY(hipEventCreate((hipEvent_t *)event));
+    return ACC_OK;
+}
+
+extern "C" int acc_event_record(void *event, void *stream) {
+    HIP_TRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return ACC_OK;
+}
+
+extern "C" int acc_event_elapsed_ms(void *start, void *end, float *ms) {
+    if (!ms) return fail(ACC_ERR_ARG, "null argument");
+    HIP_TRY(hipEventSynchronize((hipEvent_t)end));
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+    return ACC_OK;
+}
+
+extern "C" int acc_event_destroy(void *event) {
+    HIP_TRY(hipEventDestroy((hipEvent_t)event));
+    return ACC_OK;
+}
+
 extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int device, acc_gpu_matrix **out) {
     if (!out || (!csr && !pk)) return fail(ACC_ERR_ARG, "need a CSR and/or packed matrix");
     if (csr && pk && (csr->rows != pk->rows || csr->cols != pk->cols))

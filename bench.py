@@ -49,16 +49,16 @@ def parse():
     return ap.parse_args()
 
 
-def event_time_ms(torch, stream, fn, reps):
+def event_time_ms(acc, stream, fn, reps):
     """Average duration of fn() (kernels on `stream`) from HIP events on that stream."""
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    starts = [acc.Event() for _ in range(reps)]
+    ends = [acc.Event() for _ in range(reps)]
     for i in range(reps):
         starts[i].record(stream)
         fn()
         ends[i].record(stream)
-    torch.cuda.synchronize()
-    return float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    stream.synchronize()
+    return float(np.mean([s.elapsed_ms(e) for s, e in zip(starts, ends)]))
 
 
 def cpu_baseline(acc, csr, X, k, seconds):
@@ -82,27 +82,25 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the row-sharded multi-GPU "
               "design is described in DESIGN.md and is not measured here.", file=sys.stderr)
         sys.exit(2)
-    import torch
     import acceles_amd as acc
 
-    assert torch.cuda.is_available(), "no GPU visible"
-    torch.cuda.init()
-    stream = torch.cuda.current_stream()
+    assert acc.device_count() > 0, "no GPU visible"
+    stream = acc.Stream(0)
 
     t0 = time.perf_counter()
     csr = acc.CSRMatrix.generate(a.rows, a.cols, deg=tuple(a.deg), seed=0xACCE1E5 + 1)
     pk = csr.pack(bits=a.bits, split=a.split)
     g = acc.GpuMatrix(csr, pk, device=0)
     X = acc.generate_queries(a.batch, a.cols, seed=0xACCE1E5 + 2)
-    Xd = torch.from_numpy(X).cuda()
+    Xd = acc.DeviceBuffer.from_numpy(X)
     log(f"[bench] setup {time.perf_counter() - t0:.1f}s: nnz={csr.nnz} packed={pk.info.stream_bytes / 1e6:.1f}MB "
         f"({pk.info.bytes_per_nnz:.3f} B/nnz) fp32={g.info.csr_bytes / 1e6:.1f}MB group={g.info.group} "
         f"items={g.info.work_items}")
 
     B, K, CK = a.batch, a.k, a.ck
-    out = torch.empty((B, K, 2), dtype=torch.int32, device="cuda")
-    xs = [Xd[i] for i in range(B)]
-    outs = [out[i] for i in range(B)]
+    out = acc.DeviceBuffer((B, K, 2), np.int32)
+    xs = [Xd.at(i * a.cols) for i in range(B)]
+    outs = [out.at(i * K * 2) for i in range(B)]
 
     def lowbit_step():
         for i in range(B):
@@ -115,26 +113,26 @@ def main():
     # ---- timed region: low-bit two-stage ----
     for _ in range(a.warmup):
         lowbit_step()
-    torch.cuda.synchronize()
+    stream.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         lowbit_step()
-    torch.cuda.synchronize()
+    stream.synchronize()
     dt = time.perf_counter() - t0
     qps = a.steps * B / dt
     ms_step = dt * 1e3 / a.steps
     log(f"[bench] low-bit two-stage: {qps:.0f} queries/s ({ms_step:.3f} ms per {B}-query step)")
-    lowbit_res = out.cpu().numpy().copy()
+    lowbit_res = out.numpy()
 
     # ---- FP32 CSR Top-K on the same GPU ----
     for _ in range(2):
         fp32_step()
-    torch.cuda.synchronize()
+    stream.synchronize()
     fsteps = max(2, a.steps // 4)
     t0 = time.perf_counter()
     for _ in range(fsteps):
         fp32_step()
-    torch.cuda.synchronize()
+    stream.synchronize()
     fqps = fsteps * B / (time.perf_counter() - t0)
     log(f"[bench] fp32 CSR Top-K: {fqps:.0f} queries/s; low-bit speedup {qps / fqps:.2f}x")
 
@@ -143,22 +141,23 @@ def main():
     s_ms, f_ms = [], []
     for i in range(reps):
         x = xs[i % B]
-        s_ms.append(event_time_ms(torch, stream, lambda: g.lowbit_stage1(x, CK, stream), 1))
-        f_ms.append(event_time_ms(torch, stream, lambda: g.lowbit_stage2(x, K, CK, outs[0], stream), 1))
+        s_ms.append(event_time_ms(acc, stream, lambda: g.lowbit_stage1(x, CK, stream), 1))
+        f_ms.append(event_time_ms(acc, stream, lambda: g.lowbit_stage2(x, K, CK, outs[0], stream), 1))
     stage1_ms, final_ms = float(np.mean(s_ms)), float(np.mean(f_ms))
     s1_bytes = pk.info.stream_bytes + a.cols * 4
     achieved = s1_bytes / (stage1_ms * 1e-3) / 1e9
-    fp32_ms = event_time_ms(torch, stream, lambda: g.topk_fp32(xs[0], K, outs[0], stream), 20)
+    fp32_ms = event_time_ms(acc, stream, lambda: g.topk_fp32(xs[0], K, outs[0], stream), 20)
     log(f"[bench] stage1 {stage1_ms * 1e3:.1f}us ({achieved:.0f} GB/s algorithmic), stage2 {final_ms * 1e3:.1f}us, "
         f"fp32 query {fp32_ms * 1e3:.1f}us")
 
     # ---- accuracy vs FP64 exact Top-K ----
-    y64 = torch.empty(a.rows, dtype=torch.float64, device="cuda")
+    y64 = acc.DeviceBuffer((a.rows,), np.float64)
     prec = []
     allfound = []
     for i in range(B):
         g.spmv_fp64(xs[i], y64, stream)
-        yy = y64.cpu().numpy()
+        stream.synchronize()
+        yy = y64.numpy()
         top = np.argpartition(-yy, K)[:K + 1]
         exact = top[np.lexsort((top, -yy[top]))][:K]
         got = lowbit_res[i, :, 1]

@@ -184,6 +184,22 @@ int acc_generate_queries(int32_t nq, int32_t cols, int val_kind, int l2norm, uin
 
 int acc_gpu_device_count(int *count);
 
+/* Device memory, streams and events for callers without a GPU framework of
+ * their own (the cl::Buffer / enqueueMigrateMemObjects / profiling-event
+ * plumbing of host_spmv_ucsr.cpp:501-539 and opencl_utils.hpp:335-344).
+ * Copies are synchronous with respect to the host. */
+int acc_dev_malloc(int device, size_t bytes, void **ptr);
+int acc_dev_free(void *ptr);
+int acc_memcpy_h2d(void *dst_dev, const void *src_host, size_t bytes);
+int acc_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes);
+int acc_stream_create(int device, void **stream);
+int acc_stream_destroy(void *stream);
+int acc_stream_synchronize(void *stream);
+int acc_event_create(void **event);
+int acc_event_record(void *event, void *stream);
+int acc_event_elapsed_ms(void *start, void *end, float *ms); /* waits for end */
+int acc_event_destroy(void *event);
+
 /* SpMV::setup (src/gpu/host_spmv_topk_csr_gpu.cu:104-169) and Q_setup
  * (host_spmv_ucsr.cpp:501-539): copy the FP32 CSR and (optionally) the
  * low-bit packets to device memory on `device`.  Either may be NULL but not
