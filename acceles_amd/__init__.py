@@ -116,10 +116,10 @@ SIGNATURES = [
     ("acc_gpu_upload", C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
     ("acc_gpu_matrix_info", C.c_int, [_P, C.POINTER(GpuInfo)]),
     ("acc_gpu_matrix_destroy", None, [_P]),
+    ("acc_gpu_set_profiling", C.c_int, [_P, C.c_int]),
+    ("acc_gpu_stage_times", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
     ("acc_gpu_topk_fp32", C.c_int, [_P, _P, C.c_int, _P, _P]),
     ("acc_gpu_topk_lowbit", C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P]),
-    ("acc_gpu_lowbit_stage1", C.c_int, [_P, _P, C.c_int, _P]),
-    ("acc_gpu_lowbit_stage2", C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P]),
     ("acc_gpu_spmv_fp64", C.c_int, [_P, _P, _P, _P]),
     ("acc_gpu_topk_lowbit_batch", C.c_int, [_P, _P, C.c_int, _I32P, C.c_int, _P, C.c_int, _P]),
     ("acc_gpu_spmv_fp32", C.c_int, [_P, _P, _P, _P]),
@@ -483,6 +483,16 @@ class GpuMatrix:
         _check(lib().acc_gpu_matrix_info(self._h, C.byref(self.info)))
         self.rows, self.cols = self.info.rows, self.info.cols
 
+    def set_profiling(self, enable=True):
+        _check(lib().acc_gpu_set_profiling(self._h, int(enable)))
+
+    def stage_times(self):
+        """Launch durations (ms) of the last query while profiling is on."""
+        ms = (C.c_float * 8)()
+        n = C.c_int()
+        _check(lib().acc_gpu_stage_times(self._h, ms, 8, C.byref(n)))
+        return [ms[i] for i in range(n.value)]
+
     # host-pointer forms (numpy in, numpy out)
     def topk_fp32_host(self, x, k):
         out = np.empty(k, ELEMENT_DTYPE)
@@ -500,12 +510,6 @@ class GpuMatrix:
 
     def topk_lowbit(self, x, k, ck, out, stream=None):
         _check(lib().acc_gpu_topk_lowbit(self._h, _ptr(x), k, ck, _ptr(out), _stream(stream)))
-
-    def lowbit_stage1(self, x, ck, stream=None):
-        _check(lib().acc_gpu_lowbit_stage1(self._h, _ptr(x), ck, _stream(stream)))
-
-    def lowbit_stage2(self, x, k, ck, out, stream=None):
-        _check(lib().acc_gpu_lowbit_stage2(self._h, _ptr(x), k, ck, _ptr(out), _stream(stream)))
 
     def spmv_fp64(self, x, y, stream=None):
         _check(lib().acc_gpu_spmv_fp64(self._h, _ptr(x), _ptr(y), _stream(stream)))

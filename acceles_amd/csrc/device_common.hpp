@@ -46,67 +46,91 @@ __device__ __forceinline__ uint64_t atomic_load_u64(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Bitonic sort of buf[0, cap) into descending order by one wavefront.
-__device__ inline void wave_sort_desc(uint64_t *buf, int cap) {
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+    uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src);
+    uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+    uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m);
+    uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Bitonic sort, descending, of 64*R keys held in registers: element
+// i = r*64 + lane lives in v[r] of that lane.  Stages with j >= 64 exchange
+// slots inside a lane; the others exchange lanes with shfl_xor.
+template <int R>
+__device__ __forceinline__ void reg_sort_desc(uint64_t (&v)[R]) {
     const int lane = lane_id();
-    for (int k = 2; k <= cap; k <<= 1) {
+#pragma unroll
+    for (int k = 2; k <= 64 * R; k <<= 1) {
+#pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = lane; t < (cap >> 1); t += 64) {
-                int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                int l = i | j;
-                uint64_t a = buf[i], b = buf[l];
-                bool desc_block = (i & k) == 0;
-                if (desc_block ? (a < b) : (a > b)) {
-                    buf[i] = b;
-                    buf[l] = a;
+            if (j >= 64) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int rp = r ^ (j >> 6);
+                    if (rp > r) {
+                        const bool desc = ((r * 64 + lane) & k) == 0;
+                        const uint64_t a = v[r], b = v[rp];
+                        const bool sw = desc ? (a < b) : (a > b);
+                        v[r] = sw ? b : a;
+                        v[rp] = sw ? a : b;
+                    }
+                }
+            } else {
+                const bool lower = (lane & j) == 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint64_t o = shfl_xor_u64(v[r], j);
+                    const bool desc = ((r * 64 + lane) & k) == 0;
+                    const bool take_max = lower == desc;
+                    v[r] = take_max ? (v[r] > o ? v[r] : o) : (v[r] < o ? v[r] : o);
                 }
             }
-            wave_sync();
         }
     }
 }
 
-// Bitonic sort of buf[0, cap) descending by the whole workgroup.
-__device__ inline void block_sort_desc(uint64_t *buf, int cap) {
-    for (int k = 2; k <= cap; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = (int)threadIdx.x; t < (cap >> 1); t += (int)blockDim.x) {
-                int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                int l = i | j;
-                uint64_t a = buf[i], b = buf[l];
-                bool desc_block = (i & k) == 0;
-                if (desc_block ? (a < b) : (a > b)) {
-                    buf[i] = b;
-                    buf[l] = a;
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// Wavefront Top-M collector over an LDS buffer of `cap` keys
-// (cap a power of two >= M + 64).  All state is wave-uniform.  push() is
-// called by all 64 lanes together; each lane offers at most one key.
+// Wavefront Top-M collector over an LDS buffer of cap = 64*R keys
+// (M <= cap - 64).  All state is wave-uniform.  push() is called by all 64
+// lanes together; each lane offers at most one key.  Appends go to LDS; a
+// compaction reads the buffer into registers, sorts there and writes back the
+// best M.
+template <int R>
 struct WaveTopM {
+    static constexpr int cap = 64 * R;
     uint64_t *buf;
-    int n, cap, M;
+    int n, M;
     uint64_t thr;  // a key must be > thr to be kept (the M-th best once full)
 
-    __device__ void init(uint64_t *b, int cap_, int m) {
+    __device__ void init(uint64_t *b, int m) {
         buf = b;
-        cap = cap_;
         M = m;
         n = 0;
         thr = 0;
     }
     // Sort, keep the best M, set the admission threshold.
     __device__ void compact() {
-        for (int i = n + lane_id(); i < cap; i += 64) buf[i] = 0;
-        wave_sync();
-        wave_sort_desc(buf, cap);
+        const int lane = lane_id();
+        uint64_t v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = (r * 64 + lane < n) ? buf[r * 64 + lane] : 0ull;
+        reg_sort_desc<R>(v);
         n = n < M ? n : M;
-        if (n == M && M > 0) thr = buf[M - 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (r * 64 + lane < n) buf[r * 64 + lane] = v[r];
+        if (n == M && M > 0) {
+            uint64_t t = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (r == (M - 1) >> 6) t = v[r];
+            thr = shfl_u64(t, (M - 1) & 63);
+        }
+        wave_sync();
     }
     __device__ void push(bool pred, uint64_t key) {
         pred = pred && key > thr;

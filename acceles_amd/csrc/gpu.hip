@@ -1,5 +1,5 @@
 // gpu.hip -- device-resident matrices and the GPU entry points of the C ABI.
-#include "kernels.hpp"
+#include "kernels_common.hpp"
 #include "internal.hpp"
 
 #include <hip/hip_runtime.h>
@@ -21,7 +21,8 @@ using acc::fail;
     } while (0)
 
 static constexpr int kMaxM = 64;     // rows kept per partition
-static constexpr int kMaxK = 960;    // fused FP32 Top-K limit (collector cap 1024)
+static constexpr int kMaxK = 960;    // Top-K limit of the FP32 path (collector cap 1024)
+static constexpr int kMaxCK = 4096;  // candidates of the low-bit path
 
 struct acc_gpu_matrix {
     int device = 0;
@@ -33,6 +34,12 @@ struct acc_gpu_matrix {
     uint32_t *d_ptr = nullptr;
     int32_t *d_idx = nullptr;
     float *d_val = nullptr;
+    float *d_y = nullptr;            // FP32 scores of the plain path
+    // select state of the FP32 path
+    int sel_blocks = 1, sel_chunk = 8192;
+    uint64_t *d_sthr = nullptr;
+    uint32_t *d_scnt = nullptr;
+    uint64_t *d_scand = nullptr;
     // low-bit sliced packets
     bool has_packed = false;
     int bits = 8, col_bits = 10, entry_bits = 18, is_signed = 0;
@@ -40,21 +47,27 @@ struct acc_gpu_matrix {
     uint32_t *d_words = nullptr;
     uint2 *d_slices = nullptr;
     int32_t *d_perm = nullptr;
-    int4 *d_items = nullptr;
+    int4 *d_items = nullptr;         // per workgroup
     int nitems = 0;
-    int32_t *d_part_row_begin = nullptr;
     std::vector<int32_t> part_row_begin;
-    uint64_t *d_thr = nullptr;      // split
-    uint32_t *d_cnt = nullptr;      // split
+    uint64_t *d_thr = nullptr;       // split
+    uint32_t *d_cnt = nullptr;       // split
     uint64_t *d_cand = nullptr;
-    int64_t *d_cand_off = nullptr;  // split + 1
-    uint64_t *d_resc = nullptr;     // split * kMaxM
-    uint32_t *d_done = nullptr;
-    // FP32 Top-K state
-    int fp32_blocks = 0;
-    uint64_t *d_fthr = nullptr;
-    uint32_t *d_fcnt = nullptr;
-    uint64_t *d_fcand = nullptr;
+    int64_t *d_cand_off = nullptr;   // split + 1
+    uint64_t *d_resc = nullptr;      // split * kMaxM
+    int32_t *d_crow = nullptr, *d_cscore = nullptr;  // split * kMaxM (candidate outputs)
+    int64_t *d_part_slice = nullptr;                 // split + 1
+    // batched-path workspace (grown on demand, freed with the handle)
+    int32_t *d_bS = nullptr;
+    size_t bS_elems = 0;
+    uint64_t *d_bresc = nullptr;
+    size_t bresc_elems = 0;
+    int32_t *d_bK = nullptr;
+    int bK_elems = 0;
+    // profiling events around the launches of the last query
+    bool profiling = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int nev = 0;  // events recorded by the last query
     // host-call staging
     float *d_xstage = nullptr;
     acc_element *d_ostage = nullptr;
@@ -79,25 +92,63 @@ void free_all(acc_gpu_matrix *m) {
     hipSetDevice(m->device);
     for (void *p : m->allocs) hipFree(p);
     m->allocs.clear();
+    for (auto &e : m->ev)
+        if (e) {
+            hipEventDestroy(e);
+            e = nullptr;
+        }
 }
 
+// Grows a handle-owned workspace buffer to at least `need` elements.
+template <typename T>
+int grow(acc_gpu_matrix *m, T **p, size_t *cap, size_t need) {
+    if (*p && *cap >= need) return ACC_OK;
+    if (*p) {
+        hipFree(*p);
+        m->allocs.erase(std::find(m->allocs.begin(), m->allocs.end(), (void *)*p));
+        m->device_bytes -= (int64_t)(*cap * sizeof(T));
+        *p = nullptr;
+        *cap = 0;
+    }
+    int rc = dalloc(m, p, need);
+    if (rc == ACC_OK) *cap = need;
+    return rc;
+}
+
+// Records profiling event i on the query's stream (when profiling is on).
+inline void mark(acc_gpu_matrix *m, int i, hipStream_t st) {
+    if (m->profiling) {
+        hipEventRecord(m->ev[i], st);
+        m->nev = i + 1;
+    }
+}
+
+int pow2_at_least(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// G lanes per row for the FP32 dot products: a power of two near a quarter of
+// the mean row length, in [2, 32].
 int choose_group(double avg) {
     int g = 2;
-    while (g < 32 && g * 2 <= avg / 2) g <<= 1;
+    while (g < 32 && g * 4 <= avg) g <<= 1;
     return g;
 }
 
-// Wavefront work items: contiguous slice ranges inside one partition with
-// about equal dword counts.
-std::vector<int4> make_items(const acc_packed *pk, int cus) {
+// Workgroup work items: contiguous slice ranges inside one partition with
+// about equal work (dwords plus a fixed per-slice cost).
+std::vector<int4> make_items(const acc_packed *pk, int cus, std::vector<int32_t> &part_wgs) {
     const int S = pk->split;
     const int64_t NS = pk->nslices();
-    const int64_t kSliceCost = 8;  // fixed per-slice work, in dwords
+    const int64_t kSliceCost = 8;
     int64_t total = 0;
     for (int64_t s = 0; s < NS; ++s) total += pk->slice_ndw[(size_t)s] + kSliceCost;
-    int64_t target_items = (int64_t)cus * ACC_WAVES * 4;
-    int64_t per = std::max<int64_t>(total / std::max<int64_t>(target_items, 1), 256);
+    const int64_t avg_slice = std::max<int64_t>(1, total / std::max<int64_t>(NS, 1));
+    const int64_t per = std::max<int64_t>(total / ((int64_t)cus * 6), avg_slice * LB_WAVES);
     std::vector<int4> items;
+    part_wgs.assign((size_t)S, 0);
     for (int p = 0; p < S; ++p) {
         int64_t s0 = pk->part_slice_begin[(size_t)p], s1 = pk->part_slice_begin[(size_t)p + 1];
         int64_t b = s0, acc = 0;
@@ -105,6 +156,7 @@ std::vector<int4> make_items(const acc_packed *pk, int cus) {
             acc += pk->slice_ndw[(size_t)s] + kSliceCost;
             if (acc >= per || s + 1 == s1) {
                 items.push_back(make_int4(p, (int)b, (int)(s + 1), 0));
+                part_wgs[(size_t)p]++;
                 b = s + 1;
                 acc = 0;
             }
@@ -113,62 +165,8 @@ std::vector<int4> make_items(const acc_packed *pk, int cus) {
     return items;
 }
 
-int cap_for(int M) { return std::max(128, (int)[](int v) { int p = 1; while (p < v) p <<= 1; return p; }(M + 64)); }
-
-typedef void (*stage1_fn)(Stage1Args);
-
-template <int E, bool SIGNED>
-stage1_fn stage1_for_mode(int mode) {
-    return mode == 0 ? lowbit_stage1<E, SIGNED, 0> : lowbit_stage1<E, SIGNED, 1>;
-}
-
-template <bool SIGNED>
-stage1_fn stage1_for_e(int E, int mode) {
-    switch (E) {
-        case 10: return stage1_for_mode<10, SIGNED>(mode);
-        case 11: return stage1_for_mode<11, SIGNED>(mode);
-        case 12: return stage1_for_mode<12, SIGNED>(mode);
-        case 13: return stage1_for_mode<13, SIGNED>(mode);
-        case 14: return stage1_for_mode<14, SIGNED>(mode);
-        case 15: return stage1_for_mode<15, SIGNED>(mode);
-        case 16: return stage1_for_mode<16, SIGNED>(mode);
-        case 17: return stage1_for_mode<17, SIGNED>(mode);
-        case 18: return stage1_for_mode<18, SIGNED>(mode);
-        case 19: return stage1_for_mode<19, SIGNED>(mode);
-        case 20: return stage1_for_mode<20, SIGNED>(mode);
-        case 21: return stage1_for_mode<21, SIGNED>(mode);
-        case 22: return stage1_for_mode<22, SIGNED>(mode);
-        default: return nullptr;
-    }
-}
-
-stage1_fn stage1_kernel(int E, bool is_signed, int mode) {
-    return is_signed ? stage1_for_e<true>(E, mode) : stage1_for_e<false>(E, mode);
-}
-
-typedef void (*final_fn)(FinalArgs);
-final_fn final_kernel(int G) {
-    switch (G) {
-        case 2: return lowbit_final<2>;
-        case 4: return lowbit_final<4>;
-        case 8: return lowbit_final<8>;
-        case 16: return lowbit_final<16>;
-        case 32: return lowbit_final<32>;
-        default: return nullptr;
-    }
-}
-
-typedef void (*fp32_fn)(Fp32Args);
-fp32_fn fp32_kernel(int G, int mode) {
-    switch (G) {
-        case 2: return mode ? fp32_topk<2, 1> : fp32_topk<2, 0>;
-        case 4: return mode ? fp32_topk<4, 1> : fp32_topk<4, 0>;
-        case 8: return mode ? fp32_topk<8, 1> : fp32_topk<8, 0>;
-        case 16: return mode ? fp32_topk<16, 1> : fp32_topk<16, 0>;
-        case 32: return mode ? fp32_topk<32, 1> : fp32_topk<32, 0>;
-        default: return nullptr;
-    }
-}
+// Collector size R (64*R keys per wavefront) for keeping the best M: M <= 64*R - 64.
+int r_for(int M) { return std::max(2, pow2_at_least(M + 64) / 64); }
 
 int check_launch() {
     hipError_t e = hipGetLastError();
@@ -176,62 +174,107 @@ int check_launch() {
     return ACC_OK;
 }
 
-// Launches stage 1 for one query; mode 0 -> candidates, mode 1 -> scores.
-int launch_stage1(acc_gpu_matrix *m, const float *x, int M, int mode, int32_t *y, hipStream_t st) {
-    Stage1Args a;
+// One low-bit query (stage 1, per-partition merge + rescoring, final Top-K):
+// Top-K into out (K > 0) and/or the per-partition candidates into
+// crow/cscore (p*M + i).
+int launch_lowbit(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *out, int32_t *crow, int32_t *cscore,
+                  hipStream_t st) {
+    LowbitArgs a;
     a.words = m->d_words;
     a.slices = m->d_slices;
     a.perm = m->d_perm;
     a.items = m->d_items;
-    a.nitems = m->nitems;
     a.x = x;
     a.cols = m->cols;
     a.col_bits = m->col_bits;
     a.val_bits = m->bits;
+    a.split = m->split;
     a.M = M;
-    a.cap = cap_for(M);
     a.thr = m->d_thr;
     a.cnt = m->d_cnt;
     a.cand = m->d_cand;
     a.cand_off = m->d_cand_off;
-    a.y = y;
-    stage1_fn k = stage1_kernel(m->entry_bits, m->is_signed != 0, mode);
+    a.ptr = m->d_ptr;
+    a.idx = m->d_idx;
+    a.val = m->d_val;
+    a.G = m->group;
+    a.resc = m->d_resc;
+    a.K = K;
+    a.out = out;
+    a.cand_rows_out = crow;
+    a.cand_scores_out = cscore;
+    lowbit_fn k = get_lowbit_stage1(m->entry_bits, m->is_signed != 0);
     if (!k) return fail(ACC_ERR_ARG, "unsupported entry width");
-    size_t lds = 64 + (mode == 0 ? (size_t)ACC_WAVES * a.cap * 8 : 0) + (size_t)((m->cols + 15) & ~15);
-    int blocks = (m->nitems + ACC_WAVES - 1) / ACC_WAVES;
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(ACC_WG), lds, st, a);
+    const size_t coll = 8 * (size_t)LB_WAVES * 128;  // eight WaveTopM<2> buffers
+    const size_t lds1 = 64 + coll + (size_t)((m->cols + 15) & ~15);
+    mark(m, 0, st);
+    hipLaunchKernelGGL(k, dim3(m->nitems), dim3(LB_WG), lds1, st, a);
+    int rc = check_launch();
+    if (rc != ACC_OK) return rc;
+    mark(m, 1, st);
+    hipLaunchKernelGGL(get_lowbit_merge(), dim3(m->split), dim3(LB_WG), 64 + coll, st, a);
+    rc = check_launch();
+    mark(m, 2, st);
+    if (rc != ACC_OK || K <= 0) return rc;
+    const int R = r_for(K);
+    lowbit_fn f = get_lowbit_final(R);
+    if (!f) return fail(ACC_ERR_ARG, "k too large");
+    hipLaunchKernelGGL(f, dim3(1), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
+    mark(m, 3, st);
     return check_launch();
 }
 
-int launch_final(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *out, int32_t *crow, int32_t *cscore,
-                 hipStream_t st) {
-    FinalArgs f;
-    f.cnt = m->d_cnt;
-    f.cand = m->d_cand;
-    f.cand_off = m->d_cand_off;
-    f.thr = m->d_thr;
-    f.part_row_begin = m->d_part_row_begin;
-    f.split = m->split;
-    f.M = M;
-    f.cap = cap_for(M);
-    f.ptr = m->d_ptr;
-    f.idx = m->d_idx;
-    f.val = m->d_val;
-    f.x = x;
-    f.G = m->group;
-    f.resc = m->d_resc;
-    f.done = m->d_done;
-    f.K = K;
-    f.out = out;
-    f.cand_rows_out = crow;
-    f.cand_scores_out = cscore;
-    final_fn k = final_kernel(m->group);
-    if (!k) return fail(ACC_ERR_ARG, "unsupported group size");
-    int total = m->split * M;
-    int cap2 = 1;
-    while (cap2 < total) cap2 <<= 1;
-    size_t lds = std::max((size_t)ACC_WAVES * f.cap * 8 + (size_t)ACC_WAVES * M * 8, (size_t)cap2 * 8);
-    hipLaunchKernelGGL(k, dim3(m->split), dim3(ACC_WG), lds, st, f);
+// Batched low-bit queries: batch_scores over query blocks, batch_select per
+// (partition, query), batch_final per query (skipped when out is null).
+// Workspace (d_bS, d_bresc, d_bK) must already hold nq queries.
+int launch_batch(acc_gpu_matrix *m, const float *X, int nq, int M, int kmax, acc_element *out, int out_stride,
+                 int32_t *crow, int32_t *cscore, hipStream_t st) {
+    BatchArgs a;
+    a.words = m->d_words;
+    a.slices = m->d_slices;
+    a.perm = m->d_perm;
+    a.items = m->d_items;
+    a.part_slice = m->d_part_slice;
+    a.X = X;
+    a.nq = nq;
+    a.cols = m->cols;
+    a.col_bits = m->col_bits;
+    a.val_bits = m->bits;
+    a.nslots = m->nslices * 64;
+    a.S = m->d_bS;
+    a.split = m->split;
+    a.M = M;
+    a.ptr = m->d_ptr;
+    a.idx = m->d_idx;
+    a.val = m->d_val;
+    a.G = m->group;
+    a.resc = out ? m->d_bresc : nullptr;
+    a.K = m->d_bK;
+    a.out = out;
+    a.out_stride = out_stride;
+    a.cand_rows_out = crow;
+    a.cand_scores_out = cscore;
+    // query block: 32 queries per matrix pass, 16 for small batches
+    const int QB = (nq <= 16 || (size_t)m->cols * (32 + 8) > 65536) ? 16 : 32;
+    a.QB = QB;
+    batch_fn fs = QB == 16 ? get_batch_scores16(m->entry_bits, m->is_signed != 0, m->bits)
+                           : get_batch_scores32(m->entry_bits, m->is_signed != 0, m->bits);
+    if (!fs) return fail(ACC_ERR_ARG, "unsupported entry width");
+    const size_t lds_q = (size_t)m->cols * (QB + 8);
+    mark(m, 0, st);
+    hipLaunchKernelGGL(fs, dim3(m->nitems, (nq + QB - 1) / QB), dim3(ACC_WG), lds_q, st, a);
+    int rc = check_launch();
+    if (rc != ACC_OK) return rc;
+    mark(m, 1, st);
+    hipLaunchKernelGGL(get_batch_select(), dim3(m->split, nq), dim3(ACC_WG), (size_t)ACC_WAVES * 128 * 8, st, a);
+    rc = check_launch();
+    mark(m, 2, st);
+    if (rc != ACC_OK || !out) return rc;
+    const int R = r_for(kmax);
+    batch_fn ff = get_batch_final(R);
+    if (!ff) return fail(ACC_ERR_ARG, "k too large");
+    hipLaunchKernelGGL(ff, dim3(nq), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
+    mark(m, 3, st);
     return check_launch();
 }
 
@@ -241,16 +284,47 @@ int check_lowbit(acc_gpu_matrix *m, int ck, int k) {
     if (ck < m->split) return fail(ACC_ERR_ARG, "ck must be >= split");
     int M = ck / m->split;
     if (M > kMaxM) return fail(ACC_ERR_ARG, "ck/split must be <= 64");
-    if (k < 0 || k > m->split * M) return fail(ACC_ERR_ARG, "k must be in [0, (ck/split)*split]");
+    if ((int64_t)m->split * M > kMaxCK) return fail(ACC_ERR_ARG, "ck must be <= 4096");
+    if (k < 0 || k > m->split * M || k > kMaxK) return fail(ACC_ERR_ARG, "k must be in [0, min(960, (ck/split)*split)]");
     if (k > 0 && !m->has_csr) return fail(ACC_ERR_ARG, "rescoring needs the FP32 CSR (upload it too)");
-    if ((int64_t)m->split * M > 4096) return fail(ACC_ERR_ARG, "ck must be <= 4096");
     return ACC_OK;
+}
+
+// Top-K of a device score vector (select_blocks, then select_final); the
+// state must be zero on entry and is left zero.
+int launch_select(const float *y, int n, int K, int blocks, int chunk, uint64_t *thr, uint32_t *cnt, uint64_t *cand,
+                  acc_element *out, hipStream_t st, acc_gpu_matrix *prof = nullptr) {
+    SelectArgs s;
+    s.y = y;
+    s.n = n;
+    s.chunk = chunk;
+    s.K = K;
+    s.thr = thr;
+    s.cnt = cnt;
+    s.cand = cand;
+    s.out = out;
+    const int R = r_for(K);
+    select_fn fb = get_select_blocks(R), ff = get_select_final(R);
+    if (!fb || !ff) return fail(ACC_ERR_ARG, "k too large");
+    const size_t lds = (size_t)ACC_WAVES * 64 * R * 8;
+    hipLaunchKernelGGL(fb, dim3(blocks), dim3(ACC_WG), lds, st, s);
+    int rc = check_launch();
+    if (rc != ACC_OK) return rc;
+    if (prof) mark(prof, 2, st);
+    hipLaunchKernelGGL(ff, dim3(1), dim3(ACC_WG), lds, st, s);
+    if (prof) mark(prof, 3, st);
+    return check_launch();
+}
+
+void select_geometry(int n, int cus, int &blocks, int &chunk) {
+    chunk = std::max(8192, (n + 2 * cus - 1) / (2 * cus));
+    blocks = std::max(1, (n + chunk - 1) / chunk);
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// upload / info / destroy
+// device memory, streams, events
 // ---------------------------------------------------------------------------
 
 extern "C" int acc_gpu_device_count(int *count) {
@@ -324,6 +398,11 @@ extern "C" int acc_event_destroy(void *event) {
     return ACC_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// upload / info / destroy
+// ---------------------------------------------------------------------------
+
 extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int device, acc_gpu_matrix **out) {
     if (!out || (!csr && !pk)) return fail(ACC_ERR_ARG, "need a CSR and/or packed matrix");
     if (csr && pk && (csr->rows != pk->rows || csr->cols != pk->cols))
@@ -332,21 +411,40 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(ACC_ERR_NODEVICE, "no GPU visible");
     if (device < 0 || device >= ndev) return fail(ACC_ERR_ARG, "bad device index");
     HIP_TRY(hipSetDevice(device));
+    const int32_t cols = csr ? csr->cols : pk->cols;
+    if (cols > 8192) return fail(ACC_ERR_ARG, "cols must be <= 8192");
     acc_gpu_matrix *m = new acc_gpu_matrix();
     m->device = device;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) m->cus = prop.multiProcessorCount;
     m->rows = csr ? csr->rows : pk->rows;
-    m->cols = csr ? csr->cols : pk->cols;
-    if (m->cols > 8192) { delete m; return fail(ACC_ERR_ARG, "cols must be <= 8192"); }
+    m->cols = cols;
     int rc = ACC_OK;
+    std::vector<int4> items;
+    std::vector<int32_t> part_wgs;
+    std::vector<int64_t> coff;
+    std::vector<uint2> sl;
 #define TRY(x)                         \
     do {                               \
         rc = (x);                      \
         if (rc != ACC_OK) goto failed; \
     } while (0)
+#define COPY(dst, src, bytes)                                                      \
+    do {                                                                           \
+        if (hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {     \
+            rc = fail(ACC_ERR_DEVICE, "host to device copy failed");              \
+            goto failed;                                                           \
+        }                                                                          \
+    } while (0)
+#define ZERO(dst, bytes)                                                           \
+    do {                                                                           \
+        if (hipMemset(dst, 0, bytes) != hipSuccess) {                              \
+            rc = fail(ACC_ERR_DEVICE, "device memset failed");                    \
+            goto failed;                                                           \
+        }                                                                          \
+    } while (0)
     if (csr) {
-        int64_t nnz = csr->nnz();
+        const int64_t nnz = csr->nnz();
         if (nnz >= (int64_t)UINT32_MAX) { rc = fail(ACC_ERR_RANGE, "FP32 CSR needs nnz < 2^32 on the device"); goto failed; }
         m->has_csr = true;
         m->nnz = nnz;
@@ -356,21 +454,16 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         TRY(dalloc(m, &m->d_ptr, p32.size()));
         TRY(dalloc(m, &m->d_idx, (size_t)nnz));
         TRY(dalloc(m, &m->d_val, (size_t)nnz));
-        if (hipMemcpy(m->d_ptr, p32.data(), p32.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(m->d_idx, csr->idx.data(), (size_t)nnz * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(m->d_val, csr->val.data(), (size_t)nnz * 4, hipMemcpyHostToDevice) != hipSuccess) {
-            rc = fail(ACC_ERR_DEVICE, "copying the CSR to the device failed");
-            goto failed;
-        }
-        // FP32 Top-K state: candidate capacity = waves * kMaxK
-        m->fp32_blocks = std::max(1, std::min((m->rows + 255) / 256, m->cus * 4));
-        TRY(dalloc(m, &m->d_fthr, 1));
-        TRY(dalloc(m, &m->d_fcnt, 1));
-        TRY(dalloc(m, &m->d_fcand, (size_t)m->fp32_blocks * ACC_WAVES * kMaxK));
-        if (hipMemset(m->d_fthr, 0, 8) != hipSuccess || hipMemset(m->d_fcnt, 0, 4) != hipSuccess) {
-            rc = fail(ACC_ERR_DEVICE, "hipMemset failed");
-            goto failed;
-        }
+        TRY(dalloc(m, &m->d_y, (size_t)m->rows));
+        COPY(m->d_ptr, p32.data(), p32.size() * 4);
+        COPY(m->d_idx, csr->idx.data(), (size_t)nnz * 4);
+        COPY(m->d_val, csr->val.data(), (size_t)nnz * 4);
+        select_geometry(m->rows, m->cus, m->sel_blocks, m->sel_chunk);
+        TRY(dalloc(m, &m->d_sthr, 1));
+        TRY(dalloc(m, &m->d_scnt, 1));
+        TRY(dalloc(m, &m->d_scand, (size_t)m->sel_blocks * kMaxK));
+        ZERO(m->d_sthr, 8);
+        ZERO(m->d_scnt, 4);
     }
     if (pk) {
         m->has_packed = true;
@@ -382,41 +475,37 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         m->is_signed = pk->is_signed;
         m->nslices = pk->nslices();
         m->part_row_begin = pk->part_row_begin;
-        std::vector<uint2> sl((size_t)m->nslices);
-        for (int64_t s = 0; s < m->nslices; ++s) sl[(size_t)s] = make_uint2(pk->slice_off[(size_t)s], pk->slice_ndw[(size_t)s]);
-        std::vector<int4> items = make_items(pk, m->cus);
+        sl.resize((size_t)m->nslices);
+        for (int64_t s = 0; s < m->nslices; ++s)
+            sl[(size_t)s] = make_uint2(pk->slice_off[(size_t)s], pk->slice_ndw[(size_t)s]);
+        items = make_items(pk, m->cus, part_wgs);
         m->nitems = (int)items.size();
-        std::vector<int64_t> coff((size_t)m->split + 1, 0);
-        for (auto &it : items) coff[(size_t)it.x + 1] += kMaxM;
-        for (int p = 0; p < m->split; ++p) coff[(size_t)p + 1] += coff[(size_t)p];
+        coff.assign((size_t)m->split + 1, 0);
+        for (int p = 0; p < m->split; ++p) coff[(size_t)p + 1] = coff[(size_t)p] + (int64_t)part_wgs[(size_t)p] * kMaxM;
         m->stream_bytes = (int64_t)pk->words.size() * 4 + m->nslices * 8;
         TRY(dalloc(m, &m->d_words, pk->words.size()));
         TRY(dalloc(m, &m->d_slices, sl.size()));
         TRY(dalloc(m, &m->d_perm, pk->perm.size()));
         TRY(dalloc(m, &m->d_items, items.size()));
-        TRY(dalloc(m, &m->d_part_row_begin, pk->part_row_begin.size()));
         TRY(dalloc(m, &m->d_thr, (size_t)m->split));
         TRY(dalloc(m, &m->d_cnt, (size_t)m->split));
         TRY(dalloc(m, &m->d_cand, (size_t)coff[(size_t)m->split]));
         TRY(dalloc(m, &m->d_cand_off, coff.size()));
         TRY(dalloc(m, &m->d_resc, (size_t)m->split * kMaxM));
-        TRY(dalloc(m, &m->d_done, 1));
-        if (hipMemcpy(m->d_words, pk->words.data(), pk->words.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(m->d_slices, sl.data(), sl.size() * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(m->d_perm, pk->perm.data(), pk->perm.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(m->d_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(m->d_part_row_begin, pk->part_row_begin.data(), pk->part_row_begin.size() * 4,
-                      hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(m->d_cand_off, coff.data(), coff.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemset(m->d_thr, 0, (size_t)m->split * 8) != hipSuccess ||
-            hipMemset(m->d_cnt, 0, (size_t)m->split * 4) != hipSuccess ||
-            hipMemset(m->d_done, 0, 4) != hipSuccess) {
-            rc = fail(ACC_ERR_DEVICE, "copying the packets to the device failed");
-            goto failed;
-        }
+        TRY(dalloc(m, &m->d_crow, (size_t)m->split * kMaxM));
+        TRY(dalloc(m, &m->d_cscore, (size_t)m->split * kMaxM));
+        TRY(dalloc(m, &m->d_part_slice, (size_t)m->split + 1));
+        COPY(m->d_part_slice, pk->part_slice_begin.data(), ((size_t)m->split + 1) * 8);
+        COPY(m->d_words, pk->words.data(), pk->words.size() * 4);
+        COPY(m->d_slices, sl.data(), sl.size() * sizeof(uint2));
+        COPY(m->d_perm, pk->perm.data(), pk->perm.size() * 4);
+        COPY(m->d_items, items.data(), items.size() * sizeof(int4));
+        COPY(m->d_cand_off, coff.data(), coff.size() * 8);
+        ZERO(m->d_thr, (size_t)m->split * 8);
+        ZERO(m->d_cnt, (size_t)m->split * 4);
     }
     TRY(dalloc(m, &m->d_xstage, (size_t)m->cols));
-    TRY(dalloc(m, &m->d_ostage, (size_t)kMaxK + 64));
+    TRY(dalloc(m, &m->d_ostage, (size_t)kMaxCK));
     if (hipDeviceSynchronize() != hipSuccess) { rc = fail(ACC_ERR_DEVICE, "device synchronize failed"); goto failed; }
     *out = m;
     return ACC_OK;
@@ -425,6 +514,8 @@ failed:
     delete m;
     return rc;
 #undef TRY
+#undef COPY
+#undef ZERO
 }
 
 extern "C" int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info) {
@@ -451,64 +542,68 @@ extern "C" void acc_gpu_matrix_destroy(acc_gpu_matrix *m) {
 }
 
 // ---------------------------------------------------------------------------
-// queries
+// low-bit queries
 // ---------------------------------------------------------------------------
 
 extern "C" int acc_gpu_topk_lowbit(acc_gpu_matrix *m, const float *x_dev, int k, int ck, acc_element *out_dev,
                                    void *stream) {
     int rc = check_lowbit(m, ck, k);
     if (rc != ACC_OK) return rc;
-    if (!x_dev || (k > 0 && !out_dev)) return fail(ACC_ERR_ARG, "null query or output");
+    if (!x_dev || k < 1 || !out_dev) return fail(ACC_ERR_ARG, "null query or output, or k < 1");
     HIP_TRY(hipSetDevice(m->device));
-    hipStream_t st = (hipStream_t)stream;
-    int M = ck / m->split;
-    rc = launch_stage1(m, x_dev, M, 0, nullptr, st);
-    if (rc != ACC_OK) return rc;
-    return launch_final(m, x_dev, M, k, out_dev, nullptr, nullptr, st);
-}
-
-extern "C" int acc_gpu_lowbit_stage1(acc_gpu_matrix *m, const float *x_dev, int ck, void *stream) {
-    int rc = check_lowbit(m, ck, 0);
-    if (rc != ACC_OK) return rc;
-    if (!x_dev) return fail(ACC_ERR_ARG, "null query");
-    HIP_TRY(hipSetDevice(m->device));
-    return launch_stage1(m, x_dev, ck / m->split, 0, nullptr, (hipStream_t)stream);
-}
-
-extern "C" int acc_gpu_lowbit_stage2(acc_gpu_matrix *m, const float *x_dev, int k, int ck, acc_element *out_dev,
-                                     void *stream) {
-    int rc = check_lowbit(m, ck, k);
-    if (rc != ACC_OK) return rc;
-    if (!x_dev || (k > 0 && !out_dev)) return fail(ACC_ERR_ARG, "null query or output");
-    HIP_TRY(hipSetDevice(m->device));
-    return launch_final(m, x_dev, ck / m->split, k, out_dev, nullptr, nullptr, (hipStream_t)stream);
-}
-
-extern "C" int acc_gpu_spmv_fp64(acc_gpu_matrix *m, const float *x_dev, double *y_dev, void *stream) {
-    if (!m || !m->has_csr || !x_dev || !y_dev) return fail(ACC_ERR_ARG, "bad arguments");
-    HIP_TRY(hipSetDevice(m->device));
-    int blocks = std::max(1, std::min((m->rows + 31) / 32, m->cus * 8));
-    hipLaunchKernelGGL(spmv_fp64, dim3(blocks), dim3(ACC_WG), 0, (hipStream_t)stream, m->d_ptr, m->d_idx, m->d_val,
-                       x_dev, m->rows, y_dev);
-    return check_launch();
+    return launch_lowbit(m, x_dev, ck / m->split, k, out_dev, nullptr, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq, const int32_t *k_host, int ck,
                                          acc_element *out_dev, int out_stride, void *stream) {
-    if (!m || !x_dev || !k_host || nq < 0) return fail(ACC_ERR_ARG, "bad batch arguments");
+    if (!m || !x_dev || !k_host || nq < 0 || !out_dev) return fail(ACC_ERR_ARG, "bad batch arguments");
+    if (nq == 0) return ACC_OK;
+    int kmax = 1;
     for (int q = 0; q < nq; ++q) {
-        if (k_host[q] > out_stride) return fail(ACC_ERR_ARG, "k exceeds out_stride");
-        int rc = acc_gpu_topk_lowbit(m, x_dev + (size_t)q * m->cols, k_host[q], ck, out_dev + (size_t)q * out_stride,
-                                     stream);
-        if (rc != ACC_OK) return rc;
+        if (k_host[q] < 1 || k_host[q] > out_stride) return fail(ACC_ERR_ARG, "each k must be in [1, out_stride]");
+        kmax = std::max(kmax, (int)k_host[q]);
     }
-    return ACC_OK;
+    int rc = check_lowbit(m, ck, kmax);
+    if (rc != ACC_OK) return rc;
+    HIP_TRY(hipSetDevice(m->device));
+    hipStream_t st = (hipStream_t)stream;
+    if ((size_t)m->cols * (16 + 8) > 65536) {
+        // the query table of a block does not fit in LDS: one query per pass
+        for (int q = 0; q < nq && rc == ACC_OK; ++q)
+            rc = launch_lowbit(m, x_dev + (size_t)q * m->cols, ck / m->split, k_host[q],
+                               out_dev + (size_t)q * out_stride, nullptr, nullptr, st);
+        return rc;
+    }
+    const int M = ck / m->split;
+    const size_t nslots = (size_t)m->nslices * 64;
+    rc = grow(m, &m->d_bS, &m->bS_elems, (size_t)nq * nslots);
+    if (rc == ACC_OK) rc = grow(m, &m->d_bresc, &m->bresc_elems, (size_t)nq * m->split * M);
+    size_t kcap = (size_t)m->bK_elems;
+    if (rc == ACC_OK) rc = grow(m, &m->d_bK, &kcap, (size_t)nq);
+    m->bK_elems = (int)kcap;
+    if (rc != ACC_OK) return rc;
+    HIP_TRY(hipMemcpyAsync(m->d_bK, k_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+    return launch_batch(m, x_dev, nq, M, kmax, out_dev, out_stride, nullptr, nullptr, st);
 }
 
 extern "C" int acc_gpu_scores_lowbit(acc_gpu_matrix *m, const float *x_dev, int32_t *y_dev, void *stream) {
     if (!m || !m->has_packed || !x_dev || !y_dev) return fail(ACC_ERR_ARG, "bad arguments");
     HIP_TRY(hipSetDevice(m->device));
-    return launch_stage1(m, x_dev, 1, 1, y_dev, (hipStream_t)stream);
+    ScoresArgs a;
+    a.words = m->d_words;
+    a.slices = m->d_slices;
+    a.perm = m->d_perm;
+    a.nslices = m->nslices;
+    a.x = x_dev;
+    a.cols = m->cols;
+    a.col_bits = m->col_bits;
+    a.val_bits = m->bits;
+    a.y = y_dev;
+    scores_fn k = get_lowbit_scores(m->entry_bits, m->is_signed != 0);
+    if (!k) return fail(ACC_ERR_ARG, "unsupported entry width");
+    int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((m->nslices + ACC_WAVES - 1) / ACC_WAVES, m->cus * 8));
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(ACC_WG), 64 + (size_t)((m->cols + 15) & ~15), (hipStream_t)stream, a);
+    return check_launch();
 }
 
 extern "C" int acc_gpu_candidates_lowbit(acc_gpu_matrix *m, const float *x_dev, int ck, int32_t *rows_dev,
@@ -518,53 +613,49 @@ extern "C" int acc_gpu_candidates_lowbit(acc_gpu_matrix *m, const float *x_dev, 
     if (!x_dev || !rows_dev || !scores_dev) return fail(ACC_ERR_ARG, "null argument");
     HIP_TRY(hipSetDevice(m->device));
     hipStream_t st = (hipStream_t)stream;
-    int M = ck / m->split;
-    // stage-1 candidates land at p*M + i; compact to the C-ABI layout afterwards
-    int32_t *tmp_r = nullptr, *tmp_s = nullptr;
-    HIP_TRY(hipMallocAsync((void **)&tmp_r, (size_t)m->split * M * 4, st));
-    HIP_TRY(hipMallocAsync((void **)&tmp_s, (size_t)m->split * M * 4, st));
-    rc = launch_stage1(m, x_dev, M, 0, nullptr, st);
-    if (rc == ACC_OK) rc = launch_final(m, x_dev, M, 0, nullptr, tmp_r, tmp_s, st);
+    const int M = ck / m->split;
+    rc = launch_lowbit(m, x_dev, M, 0, nullptr, m->d_crow, m->d_cscore, st);
     int total = 0;
-    if (rc == ACC_OK) {
-        for (int p = 0; p < m->split; ++p) {
-            int n = std::min(M, m->part_row_begin[(size_t)p + 1] - m->part_row_begin[(size_t)p]);
-            if (n > 0) {
-                if (hipMemcpyAsync(rows_dev + total, tmp_r + (size_t)p * M, (size_t)n * 4, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-                    hipMemcpyAsync(scores_dev + total, tmp_s + (size_t)p * M, (size_t)n * 4, hipMemcpyDeviceToDevice, st) != hipSuccess) {
-                    rc = fail(ACC_ERR_DEVICE, "copy of candidates failed");
-                    break;
-                }
-            }
-            total += n;
-        }
+    for (int p = 0; rc == ACC_OK && p < m->split; ++p) {
+        const int n = std::min(M, m->part_row_begin[(size_t)p + 1] - m->part_row_begin[(size_t)p]);
+        if (n > 0 && (hipMemcpyAsync(rows_dev + total, m->d_crow + (size_t)p * M, (size_t)n * 4, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+                      hipMemcpyAsync(scores_dev + total, m->d_cscore + (size_t)p * M, (size_t)n * 4, hipMemcpyDeviceToDevice, st) != hipSuccess))
+            rc = fail(ACC_ERR_DEVICE, "copy of candidates failed");
+        total += n;
     }
-    hipFreeAsync(tmp_r, st);
-    hipFreeAsync(tmp_s, st);
     if (rc == ACC_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(ACC_ERR_DEVICE, "stream synchronize failed");
     if (count_host) *count_host = total;
     return rc;
 }
 
-static int launch_fp32(acc_gpu_matrix *m, const float *x_dev, int k, int mode, float *y, hipStream_t st) {
-    Fp32Args a;
-    a.ptr = m->d_ptr;
-    a.idx = m->d_idx;
-    a.val = m->d_val;
-    a.x = x_dev;
-    a.rows = m->rows;
-    a.cols = m->cols;
-    a.M = k;
-    a.cap = cap_for(k);
-    a.thr = m->d_fthr;
-    a.cnt = m->d_fcnt;
-    a.cand = m->d_fcand;
-    a.y = y;
-    fp32_fn kern = fp32_kernel(m->group, mode);
-    if (!kern) return fail(ACC_ERR_ARG, "unsupported group size");
-    size_t lds = (mode == 0 ? (size_t)ACC_WAVES * a.cap * 8 : 0) + (size_t)m->cols * 4;
-    hipLaunchKernelGGL(kern, dim3(m->fp32_blocks), dim3(ACC_WG), lds, st, a);
+extern "C" int acc_gpu_spmv_fp64(acc_gpu_matrix *m, const float *x_dev, double *y_dev, void *stream) {
+    if (!m || !m->has_csr || !x_dev || !y_dev) return fail(ACC_ERR_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(m->device));
+    int blocks = std::max(1, std::min((m->rows + 31) / 32, m->cus * 8));
+    hipLaunchKernelGGL(get_spmv_fp64(), dim3(blocks), dim3(ACC_WG), 0, (hipStream_t)stream, m->d_ptr, m->d_idx, m->d_val,
+                       x_dev, m->rows, y_dev);
     return check_launch();
+}
+
+// ---------------------------------------------------------------------------
+// FP32 CSR path
+// ---------------------------------------------------------------------------
+
+static int launch_spmv_fp32(acc_gpu_matrix *m, const float *x_dev, float *y, hipStream_t st) {
+    spmv_fn k = get_fp32_spmv(m->group);
+    if (!k) return fail(ACC_ERR_ARG, "unsupported group size");
+    const int rpi = 64 / m->group;
+    const int nblocks = (m->rows + rpi - 1) / rpi;
+    const int blocks = std::max(1, std::min((nblocks + ACC_WAVES * 4 - 1) / (ACC_WAVES * 4), m->cus * 8));
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(ACC_WG), (size_t)m->cols * 4, st, m->d_ptr, m->d_idx, m->d_val, x_dev,
+                       m->rows, m->cols, y);
+    return check_launch();
+}
+
+extern "C" int acc_gpu_spmv_fp32(acc_gpu_matrix *m, const float *x_dev, float *y_dev, void *stream) {
+    if (!m || !m->has_csr || !x_dev || !y_dev) return fail(ACC_ERR_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(m->device));
+    return launch_spmv_fp32(m, x_dev, y_dev, (hipStream_t)stream);
 }
 
 extern "C" int acc_gpu_topk_fp32(acc_gpu_matrix *m, const float *x_dev, int k, acc_element *out_dev, void *stream) {
@@ -572,18 +663,34 @@ extern "C" int acc_gpu_topk_fp32(acc_gpu_matrix *m, const float *x_dev, int k, a
     if (!x_dev || !out_dev || k < 1 || k > kMaxK) return fail(ACC_ERR_ARG, "k must be in [1, 960]");
     HIP_TRY(hipSetDevice(m->device));
     hipStream_t st = (hipStream_t)stream;
-    int rc = launch_fp32(m, x_dev, k, 0, nullptr, st);
+    mark(m, 0, st);
+    int rc = launch_spmv_fp32(m, x_dev, m->d_y, st);
     if (rc != ACC_OK) return rc;
-    int cap = cap_for(k);
-    size_t lds = (size_t)ACC_WAVES * cap * 8 + (size_t)ACC_WAVES * k * 8;
-    hipLaunchKernelGGL(topk_final, dim3(1), dim3(ACC_WG), lds, st, m->d_fcnt, m->d_fcand, m->d_fthr, k, cap, out_dev);
-    return check_launch();
+    mark(m, 1, st);
+    return launch_select(m->d_y, m->rows, k, m->sel_blocks, m->sel_chunk, m->d_sthr, m->d_scnt, m->d_scand,
+                         out_dev, st, m);
 }
 
-extern "C" int acc_gpu_spmv_fp32(acc_gpu_matrix *m, const float *x_dev, float *y_dev, void *stream) {
-    if (!m || !m->has_csr || !x_dev || !y_dev) return fail(ACC_ERR_ARG, "bad arguments");
+extern "C" int acc_gpu_set_profiling(acc_gpu_matrix *m, int enable) {
+    if (!m) return fail(ACC_ERR_ARG, "null matrix");
     HIP_TRY(hipSetDevice(m->device));
-    return launch_fp32(m, x_dev, 1, 1, y_dev, (hipStream_t)stream);
+    if (enable)
+        for (auto &e : m->ev)
+            if (!e) HIP_TRY(hipEventCreate(&e));
+    m->profiling = enable != 0;
+    m->nev = 0;
+    return ACC_OK;
+}
+
+extern "C" int acc_gpu_stage_times(acc_gpu_matrix *m, float *ms, int n, int *count) {
+    if (!m || (!ms && n > 0)) return fail(ACC_ERR_ARG, "null argument");
+    if (!m->profiling || m->nev < 2) return fail(ACC_ERR_ARG, "no profiled query (enable profiling first)");
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipEventSynchronize(m->ev[m->nev - 1]));
+    int c = 0;
+    for (int i = 0; i + 1 < m->nev && c < n; ++i, ++c) HIP_TRY(hipEventElapsedTime(&ms[c], m->ev[i], m->ev[i + 1]));
+    if (count) *count = c;
+    return ACC_OK;
 }
 
 extern "C" int acc_gpu_top_k_heap(const float *y_dev, int n, int k, acc_element *out_dev, void *stream) {
@@ -594,23 +701,16 @@ extern "C" int acc_gpu_top_k_heap(const float *y_dev, int n, int k, acc_element 
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-    int blocks = std::max(1, std::min((n + 255) / 256, cus * 2));
+    int blocks = 1, chunk = 8192;
+    select_geometry(n, cus, blocks, chunk);
     uint64_t *thr = nullptr, *cand = nullptr;
     uint32_t *cnt = nullptr;
-    size_t ncand = (size_t)blocks * ACC_WAVES * k;
     HIP_TRY(hipMallocAsync((void **)&thr, 8, st));
     HIP_TRY(hipMallocAsync((void **)&cnt, 4, st));
-    HIP_TRY(hipMallocAsync((void **)&cand, ncand * 8, st));
-    HIP_TRY(hipMemsetAsync(thr, 0, 8, st));
-    HIP_TRY(hipMemsetAsync(cnt, 0, 4, st));
-    int cap = cap_for(k);
-    hipLaunchKernelGGL(dense_topk, dim3(blocks), dim3(ACC_WG), (size_t)ACC_WAVES * cap * 8, st, y_dev, n, k, cap, thr, cnt, cand);
+    HIP_TRY(hipMallocAsync((void **)&cand, (size_t)blocks * k * 8, st));
+    hipLaunchKernelGGL(get_select_reset(), dim3(1), dim3(64), 0, st, thr, cnt);
     int rc = check_launch();
-    if (rc == ACC_OK) {
-        size_t lds = (size_t)ACC_WAVES * cap * 8 + (size_t)ACC_WAVES * k * 8;
-        hipLaunchKernelGGL(topk_final, dim3(1), dim3(ACC_WG), lds, st, cnt, cand, thr, k, cap, out_dev);
-        rc = check_launch();
-    }
+    if (rc == ACC_OK) rc = launch_select(y_dev, n, k, blocks, chunk, thr, cnt, cand, out_dev, st);
     hipFreeAsync(thr, st);
     hipFreeAsync(cnt, st);
     hipFreeAsync(cand, st);

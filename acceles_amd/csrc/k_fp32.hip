@@ -1,0 +1,178 @@
+// k_fp32.hip -- plain FP32 CSR path and FP64 exact scores:
+//   fp32_spmv       FP32 CSR SpMV (cpu_mkl_baseline.cpp:261);
+//   select_blocks / select_final
+//                   Top-K of a score vector (cpu_mkl_baseline.cpp:128-149);
+//                   with fp32_spmv it is the plain FP32 CSR Top-K path
+//                   (host_spmv_topk_csr_gpu.cu:171-231 is the GPU spec);
+//   spmv_fp64       FP64 exact scores for precision/recall reporting.
+#include "kernels_common.hpp"
+
+// ---------------------------------------------------------------------------
+// FP32 CSR path
+// ---------------------------------------------------------------------------
+
+// y = A x with G lanes per row.  Lane g of a row sums its entries
+// g, g+G, g+2G, ... in that order with fmaf (the loads of up to U of them are
+// issued together), then an xor butterfly of plain adds.  Each wavefront owns
+// a contiguous range of rows.
+template <int G>
+__global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const int32_t *idx, const float *val,
+                                                   const float *x, int rows, int cols, float *y) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *xs = (float *)smem;
+    for (int c = threadIdx.x; c < cols; c += ACC_WG) xs[c] = x[c];
+    __syncthreads();
+    constexpr int RPI = 64 / G;  // rows per wavefront iteration
+    constexpr int U = 4;
+    const int lane = lane_id(), g = lane & (G - 1);
+    const int nblocks = (rows + RPI - 1) / RPI;
+    const int gw = blockIdx.x * ACC_WAVES + (threadIdx.x >> 6), nw = gridDim.x * ACC_WAVES;
+    const int per = (nblocks + nw - 1) / nw;
+    const int b0 = gw * per, b1 = min(nblocks, b0 + per);
+    if (b0 >= b1) return;
+    int r = b0 * RPI + lane / G;
+    uint32_t jb = r < rows ? ptr[r] : 0u, je = r < rows ? ptr[r + 1] : 0u;
+    for (int b = b0; b < b1; ++b) {
+        const int rn = r + RPI;
+        uint32_t nb = 0u, ne = 0u;
+        if (b + 1 < b1 && rn < rows) {  // next rows' bounds, fetched early
+            nb = ptr[rn];
+            ne = ptr[rn + 1];
+        }
+        float p = 0.f;
+        for (uint32_t j = jb + g; j < je; j += U * G) {
+            float v[U];
+            int c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t jj = j + u * G;
+                v[u] = jj < je ? __builtin_nontemporal_load(val + jj) : 0.f;
+                c[u] = jj < je ? __builtin_nontemporal_load(idx + jj) : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (j + u * G < je) p = __builtin_fmaf(v[u], xs[c[u]], p);
+        }
+#pragma unroll
+        for (int s = G / 2; s >= 1; s >>= 1) p = p + __shfl_xor(p, s);
+        if (g == 0 && r < rows) y[r] = p;
+        r = rn;
+        jb = nb;
+        je = ne;
+    }
+}
+
+
+template <int R>
+__global__ __launch_bounds__(ACC_WG) void select_blocks(SelectArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_n[ACC_WAVES];
+    uint64_t *colbuf = (uint64_t *)smem;
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    const int64_t b0 = (int64_t)blockIdx.x * a.chunk;
+    const int64_t b1 = min((int64_t)a.n, b0 + a.chunk);
+    const int64_t q = (b1 - b0 + ACC_WAVES - 1) / ACC_WAVES;
+    const int64_t w0 = b0 + wave * q, w1 = min(b1, w0 + q);
+    WaveTopM<R> col;
+    col.init(colbuf + wave * WaveTopM<R>::cap, a.K);
+    uint64_t g = atomic_load_u64(a.thr);
+    int step = 0;
+    for (int64_t i0 = w0; i0 < w1; i0 += 64) {
+        const int64_t i = i0 + lane;
+        const bool valid = i < w1;
+        const uint64_t key = valid ? dkey_f32(__builtin_nontemporal_load(a.y + i), (int32_t)i) : 0ull;
+        const uint64_t t = col.thr > g ? col.thr : g;
+        col.push(valid && key > t, key);
+        if ((++step & 15) == 0) g = atomic_load_u64(a.thr);
+    }
+    col.compact();
+    if (lane == 0) s_n[wave] = col.n;
+    __syncthreads();
+    if (wave == 0) {
+        wg_merge(col, colbuf, s_n, ACC_WAVES);
+        publish_and_append(col, a.thr, a.cnt, a.cand);
+    }
+}
+
+// Second launch (one workgroup): the final K of the appended candidates; the
+// state is left zeroed for the next query.
+template <int R>
+__global__ __launch_bounds__(ACC_WG) void select_final(SelectArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_n[ACC_WAVES];
+    uint64_t *colbuf = (uint64_t *)smem;
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t n = *a.cnt;
+    WaveTopM<R> col;
+    col.init(colbuf + wave * WaveTopM<R>::cap, a.K);
+    for (uint32_t b = wave * 64; b < n; b += ACC_WG) {
+        const uint32_t i = b + lane;
+        col.push(i < n, i < n ? a.cand[i] : 0ull);
+    }
+    col.compact();
+    if (lane == 0) s_n[wave] = col.n;
+    __syncthreads();
+    if (wave == 0) {
+        wg_merge(col, colbuf, s_n, ACC_WAVES);
+        write_results(col, a.K, a.out);
+        if (lane == 0) {
+            *a.cnt = 0u;
+            *a.thr = 0ull;
+        }
+    }
+}
+
+// Zeroes the select state (a kernel rather than a memset, so the launch
+// sequence can be captured in a graph).
+__global__ void select_reset(uint64_t *thr, uint32_t *cnt) {
+    if (threadIdx.x == 0) {
+        *thr = 0ull;
+        *cnt = 0u;
+    }
+}
+
+// FP64 exact scores: 8 lanes per row, FP64 products and sums.
+
+__global__ __launch_bounds__(ACC_WG) void spmv_fp64(const uint32_t *ptr, const int32_t *idx, const float *val,
+                                                    const float *x, int rows, double *y) {
+    const int lane8 = threadIdx.x & 7;
+    for (int r = (blockIdx.x * ACC_WG + threadIdx.x) >> 3; r < ((rows + 31) & ~31); r += (gridDim.x * ACC_WG) >> 3) {
+        double p = 0.0;
+        if (r < rows)
+            for (uint32_t j = ptr[r] + lane8; j < ptr[r + 1]; j += 8) p += (double)val[j] * (double)x[idx[j]];
+        for (int s = 4; s >= 1; s >>= 1) p += __shfl_xor(p, s);
+        if (r < rows && lane8 == 0) y[r] = p;
+    }
+}
+
+
+spmv_fn get_fp32_spmv(int G) {
+    switch (G) {
+        case 2: return fp32_spmv<2>;
+        case 4: return fp32_spmv<4>;
+        case 8: return fp32_spmv<8>;
+        case 16: return fp32_spmv<16>;
+        case 32: return fp32_spmv<32>;
+        default: return nullptr;
+    }
+}
+select_fn get_select_blocks(int R) {
+    switch (R) {
+        case 2: return select_blocks<2>;
+        case 4: return select_blocks<4>;
+        case 8: return select_blocks<8>;
+        case 16: return select_blocks<16>;
+        default: return nullptr;
+    }
+}
+select_fn get_select_final(int R) {
+    switch (R) {
+        case 2: return select_final<2>;
+        case 4: return select_final<4>;
+        case 8: return select_final<8>;
+        case 16: return select_final<16>;
+        default: return nullptr;
+    }
+}
+reset_fn get_select_reset() { return select_reset; }
+fp64_fn get_spmv_fp64() { return spmv_fp64; }

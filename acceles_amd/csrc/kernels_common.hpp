@@ -1,0 +1,265 @@
+// kernels_common.hpp -- device helpers, argument blocks and kernel getters
+// shared by the HIP translation units of the Top-K SpMV path.
+//
+// A low-bit query is three launches on one stream:
+//   lowbit_stage1   query quantization (quantify_util.py:31-49, 259) fused
+//                   with the INT32 SpMV over sliced packets, a per-wavefront
+//                   Top-M, the workgroup merge and the append of what can
+//                   still be in the partition's Top-M (quantify_util.py:173-196;
+//                   spmv_ucsr_top_k_multicore.hpp:195-413 on the FPGA);
+//   lowbit_merge    per partition: its Top-M (host_spmv_ucsr.cpp:691-748) and
+//                   the FP32 rescoring of those rows (quantify_util.py:354-367);
+//   lowbit_final    the global Top-K of the rescored candidates.
+// Cross-workgroup data moves only through atomics and kernel boundaries: a
+// device-scope fence on a multi-XCD part writes back and invalidates the
+// whole L2, which costs microseconds per workgroup.
+//   lowbit_scores   INT32 scores of every row (parity checks).
+//   fp32_spmv       plain FP32 CSR SpMV (cpu_mkl_baseline.cpp:261).
+//   select_blocks / select_final
+//                   Top-K of a score vector (cpu_mkl_baseline.cpp:128-149);
+//                   with fp32_spmv it is the plain FP32 CSR Top-K path
+//                   (host_spmv_topk_csr_gpu.cu:171-231 is the GPU spec).
+//   spmv_fp64       FP64 exact scores for precision/recall reporting.
+#pragma once
+
+#include "device_common.hpp"
+#include "../../include/acceles.h"
+
+#define LB_WAVES 8
+#define LB_WG (LB_WAVES * 64)
+
+// Quantize the FP32 query into LDS: q = clip(rint(127 * (x / max|x|)), -128, 127)
+// (quantify_vector with B=8, symmetric; float32 division and product, half-even).
+__device__ inline void quantize_query_lds(const float *x, int cols, int8_t *qx, float *red) {
+    float m = 0.f;
+    for (int c = threadIdx.x; c < cols; c += blockDim.x) m = fmaxf(m, fabsf(x[c]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (lane_id() == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    float s = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) s = fmaxf(s, red[w]);
+    for (int c = threadIdx.x; c < cols; c += blockDim.x) {
+        int q = 0;
+        if (s > 0.f) {
+            float t = __fdiv_rn(x[c], s);
+            float u = __fmul_rn(127.0f, t);
+            q = (int)__builtin_rintf(u);
+            q = q > 127 ? 127 : (q < -128 ? -128 : q);
+        }
+        qx[c] = (int8_t)q;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// sliced-packet decoding
+// ---------------------------------------------------------------------------
+
+// Loads one group (E dwords per lane = up to 32 entries); dwords >= nd are 0.
+template <int E>
+__device__ __forceinline__ void load_group(uint32_t (&w)[E], const uint32_t *lanebase, int nd) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) w[i] = (i < nd) ? __builtin_nontemporal_load(lanebase + i * 64) : 0u;
+}
+
+// Accumulates value * q[col] over the entries of one group.  Padding entries
+// are all-zero bits (value 0), so only whole blocks of 8 padding entries are
+// skipped.
+template <int E, bool SIGNED>
+__device__ __forceinline__ int32_t decode_group(const uint32_t (&w)[E], int nd, const int8_t *qx, int cb, int vb,
+                                                uint32_t cmask, int32_t acc) {
+#pragma unroll
+    for (int blk = 0; blk < 4; ++blk) {
+        if (((blk * 8 * E) >> 5) >= nd) continue;  // whole block is padding (wave-uniform)
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const int pos = (blk * 8 + jj) * E;
+            const int wi = pos >> 5;
+            const int sh = pos & 31;
+            const uint32_t hi = (wi + 1 < E) ? w[wi + 1] : 0u;
+            const uint32_t bits = (sh + E <= 32) ? (w[wi] >> sh) : __builtin_amdgcn_alignbit(hi, w[wi], sh);
+            const int col = (int)(bits & cmask);
+            const int val = SIGNED ? __builtin_amdgcn_sbfe(bits, cb, vb) : (int)__builtin_amdgcn_ubfe(bits, cb, vb);
+            acc += val * (int)qx[col];
+        }
+    }
+    return acc;
+}
+
+// After each wave compacted its collector (list at colbuf + w*cap, count in
+// s_n[w]), wave 0 folds the other waves' lists into its own.
+template <int R>
+__device__ inline void wg_merge(WaveTopM<R> &col, const uint64_t *colbuf, const int *s_n, int waves) {
+    const int lane = lane_id();
+    for (int w = 1; w < waves; ++w) {
+        const uint64_t *src = colbuf + w * WaveTopM<R>::cap;
+        const int n = s_n[w];
+        for (int b = 0; b < n; b += 64) {
+            const int i = b + lane;
+            const bool ok = i < n;
+            col.push(ok, ok ? src[i] : 0ull);
+        }
+    }
+    col.compact();
+}
+
+// Wave 0: publish the list's M-th key as a lower bound, then append the keys
+// that can still be in the final Top-M to the shared candidate list.
+template <int R>
+__device__ inline void publish_and_append(WaveTopM<R> &col, uint64_t *thr, uint32_t *cnt, uint64_t *dst) {
+    const int lane = lane_id();
+    if (col.n >= col.M && col.M > 0 && lane == 0) atomicMax((unsigned long long *)thr, (unsigned long long)col.buf[col.M - 1]);
+    const uint64_t g = atomic_load_u64(thr);
+    int keep = 0;
+    for (int i = lane; i < col.n; i += 64) keep += col.buf[i] >= g;
+    for (int o = 32; o > 0; o >>= 1) keep += __shfl_xor(keep, o);
+    if (keep == 0) return;
+    uint32_t pos = 0;
+    if (lane == 0) pos = atomicAdd(cnt, (uint32_t)keep);
+    pos = __shfl(pos, 0);
+    for (int i = lane; i < keep; i += 64) dst[pos + i] = col.buf[i];
+}
+
+// Dot product of CSR row r with x by a group of G lanes (runtime G): lane g
+// sums entries g, g+G, ... with fmaf, then an xor butterfly of plain adds.
+__device__ __forceinline__ float group_dot_rt(const uint32_t *ptr, const int32_t *idx, const float *val, const float *x,
+                                              int r, bool valid, int G) {
+    const int g = lane_id() & (G - 1);
+    float p = 0.f;
+    if (valid) {
+        const uint32_t e = ptr[r + 1];
+        for (uint32_t j = ptr[r] + g; j < e; j += G) p = __builtin_fmaf(val[j], x[idx[j]], p);
+    }
+    for (int s = G >> 1; s >= 1; s >>= 1) p = p + __shfl_xor(p, s);
+    return p;
+}
+
+
+template <int R>
+__device__ inline void write_results(const WaveTopM<R> &col, int K, acc_element *out) {
+    for (int i = lane_id(); i < K; i += 64) {
+        const uint64_t k = i < col.n ? col.buf[i] : 0ull;
+        acc_element e;
+        e.value = k ? dkey_score_f32(k) : -__builtin_inff();
+        e.index = k ? dkey_row(k) : -1;
+        out[i] = e;
+    }
+}
+
+// One workgroup, after lowbit_merge: the global Top-K of the rescored
+
+struct LowbitArgs {
+    const uint32_t *words;      // lane-interleaved packet dwords
+    const uint2 *slices;        // (offset in 64-word units, dwords per lane)
+    const int32_t *perm;        // slot -> row (-1: padding lane)
+    const int4 *items;          // per workgroup: (partition, first slice, end slice, 0)
+    const float *x;             // FP32 query (cols)
+    int cols, col_bits, val_bits;
+    int split, M;               // partitions, rows kept per partition
+    uint64_t *thr;              // per partition: lower bound of its M-th key
+    uint32_t *cnt;              // per partition: candidate list length
+    uint64_t *cand;             // candidate lists
+    const int64_t *cand_off;    // per partition offset into cand
+    const uint32_t *ptr;        // FP32 CSR for rescoring
+    const int32_t *idx;
+    const float *val;
+    int G;
+    uint64_t *resc;             // split * M rescored keys
+    int K;                      // final Top-K (0: candidates only)
+    acc_element *out;
+    int32_t *cand_rows_out;     // optional stage-1 candidates at p*M + i
+    int32_t *cand_scores_out;
+};
+
+struct ScoresArgs {
+    const uint32_t *words;
+    const uint2 *slices;
+    const int32_t *perm;
+    int64_t nslices;
+    const float *x;
+    int cols, col_bits, val_bits;
+    int32_t *y;
+};
+
+
+struct SelectArgs {
+    const float *y;
+    int n, chunk, K;
+    uint64_t *thr;   // [1] lower bound of the K-th key
+    uint32_t *cnt;   // [1]
+    uint64_t *cand;  // gridDim.x * K
+    acc_element *out;
+};
+
+// Top-K of y[0, n), first launch: workgroup b scans [b*chunk, (b+1)*chunk)
+// (each wavefront a contiguous quarter) with wavefront collectors, merges
+// them and appends what can still be in the Top-K.
+
+// ---------------------------------------------------------------------------
+// batched low-bit path: a block of QB queries per matrix pass
+// ---------------------------------------------------------------------------
+//   batch_scores   quantizes QB queries into an LDS table (per query its own
+//                  scale) and computes INT32 scores of every slot for all QB
+//                  queries while decoding each packet entry once; scores go to
+//                  S[q][slot] (slot order, coalesced).  MACs are v_dot4 with
+//                  the value placed in one byte lane: dot4(q[c][b..b+3],
+//                  v << 8k) = q[c][b+k] * v.
+//   batch_select   per (query, partition): the partition's Top-M from S and
+//                  the FP32 rescoring of those rows.
+//   batch_final    per query: its Top-K of the rescored candidates.
+
+struct BatchArgs {
+    const uint32_t *words;
+    const uint2 *slices;
+    const int32_t *perm;
+    const int4 *items;           // batch_scores: per workgroup slice ranges
+    const int64_t *part_slice;   // split + 1 slice offsets
+    const float *X;              // nq x cols FP32 queries
+    int nq, cols, col_bits, val_bits, QB;
+    int64_t nslots;
+    int32_t *S;                  // nq x nslots INT32 scores
+    int split, M;
+    const uint32_t *ptr;         // FP32 CSR for rescoring
+    const int32_t *idx;
+    const float *val;
+    int G;
+    uint64_t *resc;              // nq x split*M rescored keys
+    const int32_t *K;            // per query
+    acc_element *out;            // per query at q * out_stride
+    int out_stride;
+    int32_t *cand_rows_out;      // optional: nq x split*M stage-1 candidates
+    int32_t *cand_scores_out;
+};
+
+// Query table layout in LDS: row c holds the QB int8 values of column c
+// (offset by +128 as unsigned bytes when U8), padded to QS bytes.
+template <int QB>
+struct QTable {
+    static constexpr int QS = QB + 8;
+};
+
+
+// ---------------------------------------------------------------------------
+// host-side getters of the kernel instantiations (one translation unit each)
+// ---------------------------------------------------------------------------
+typedef void (*lowbit_fn)(LowbitArgs);
+typedef void (*scores_fn)(ScoresArgs);
+typedef void (*select_fn)(SelectArgs);
+typedef void (*batch_fn)(BatchArgs);
+typedef void (*spmv_fn)(const uint32_t *, const int32_t *, const float *, const float *, int, int, float *);
+typedef void (*reset_fn)(uint64_t *, uint32_t *);
+typedef void (*fp64_fn)(const uint32_t *, const int32_t *, const float *, const float *, int, double *);
+
+lowbit_fn get_lowbit_stage1(int E, bool is_signed);  // k_lowbit.hip
+lowbit_fn get_lowbit_merge();
+lowbit_fn get_lowbit_final(int R);
+scores_fn get_lowbit_scores(int E, bool is_signed);
+spmv_fn get_fp32_spmv(int G);                         // k_fp32.hip
+select_fn get_select_blocks(int R);
+select_fn get_select_final(int R);
+reset_fn get_select_reset();
+fp64_fn get_spmv_fp64();
+batch_fn get_batch_scores16(int E, bool is_signed, int bits);  // k_batch16.hip
+batch_fn get_batch_scores32(int E, bool is_signed, int bits);  // k_batch32.hip
+batch_fn get_batch_select();                                   // k_batch.hip
+batch_fn get_batch_final(int R);

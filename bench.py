@@ -5,12 +5,15 @@ Workload (default): 400,000 x 500 matrix, row nnz uniform in [40, 100],
 values |N(0,1)| L2-normalised per row, 8-bit unsigned packets, SPLIT=32,
 CK=512, K=100; queries dense N(0,1) of length 500 (synthetic, seeded).
 One step = one batch of 100 queries (the reference's num_test) through the
-low-bit two-stage path.  value = queries/s over the timed steps.
+batched low-bit two-stage path (one matrix pass per block of queries).
+value = queries/s over the timed steps, inputs resident in HBM.
 
-Also reported: FP32 CSR Top-K queries/s on the same GPU and queries, the
-stage-1 kernel's roofline (algorithmic bytes / HIP-event time on the launch
-stream), precision/recall@K vs an FP64 exact Top-K, and the reference CPU
-path (FP32 SpMV + heap Top-K, single thread) timed on a bounded sample.
+Also reported: the same queries one at a time (single-query path), the plain
+FP32 CSR Top-K on the same GPU, per-launch times from HIP events on the
+query stream, the rooflines of the batched scoring kernel and of the
+single-query stage-1 kernel, precision/recall@K vs an FP64 exact Top-K, and
+the reference CPU path (FP32 SpMV + heap Top-K, one thread) on a bounded
+sample.
 """
 import argparse
 import json
@@ -23,7 +26,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+INT8_PEAK_TOPS = 5000.0     # MI355X dense int8 matrix peak (no sparsity)
 KS = [1, 8, 16, 32, 50, 75, 100]
 
 
@@ -49,19 +53,7 @@ def parse():
     return ap.parse_args()
 
 
-def event_time_ms(acc, stream, fn, reps):
-    """Average duration of fn() (kernels on `stream`) from HIP events on that stream."""
-    starts = [acc.Event() for _ in range(reps)]
-    ends = [acc.Event() for _ in range(reps)]
-    for i in range(reps):
-        starts[i].record(stream)
-        fn()
-        ends[i].record(stream)
-    stream.synchronize()
-    return float(np.mean([s.elapsed_ms(e) for s, e in zip(starts, ends)]))
-
-
-def cpu_baseline(acc, csr, X, k, seconds):
+def cpu_baseline(csr, X, k, seconds):
     """Reference CPU path (cpu_mkl_baseline.cpp:249-275 semantics: FP32 SpMV then a
     min-heap Top-k), single thread, via the oracle's C restatement."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -72,8 +64,18 @@ def cpu_baseline(acc, csr, X, k, seconds):
     while n < len(X) and time.perf_counter() - t0 < seconds:
         O.top_k_spmv_f32(ptr, idx, val, X[n], k)
         n += 1
-    dt = time.perf_counter() - t0
-    return n / dt, n
+    return n / (time.perf_counter() - t0), n
+
+
+def timed(stream, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    stream.synchronize()
+    return time.perf_counter() - t0
 
 
 def main():
@@ -98,69 +100,74 @@ def main():
         f"items={g.info.work_items}")
 
     B, K, CK = a.batch, a.k, a.ck
+    ks = np.full(B, K, np.int32)
     out = acc.DeviceBuffer((B, K, 2), np.int32)
+    out1 = acc.DeviceBuffer((B, K, 2), np.int32)
     xs = [Xd.at(i * a.cols) for i in range(B)]
-    outs = [out.at(i * K * 2) for i in range(B)]
+    outs1 = [out1.at(i * K * 2) for i in range(B)]
 
-    def lowbit_step():
+    def batch_step():
+        g.topk_lowbit_batch(Xd, ks, CK, out, K, stream)
+
+    def single_step():
         for i in range(B):
-            g.topk_lowbit(xs[i], K, CK, outs[i], stream)
+            g.topk_lowbit(xs[i], K, CK, outs1[i], stream)
 
     def fp32_step():
         for i in range(B):
-            g.topk_fp32(xs[i], K, outs[i], stream)
+            g.topk_fp32(xs[i], K, outs1[i], stream)
 
-    # ---- timed region: low-bit two-stage ----
-    for _ in range(a.warmup):
-        lowbit_step()
-    stream.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        lowbit_step()
-    stream.synchronize()
-    dt = time.perf_counter() - t0
+    # ---- timed region: batched low-bit two-stage ----
+    dt = timed(stream, batch_step, a.steps, a.warmup)
     qps = a.steps * B / dt
     ms_step = dt * 1e3 / a.steps
-    log(f"[bench] low-bit two-stage: {qps:.0f} queries/s ({ms_step:.3f} ms per {B}-query step)")
-    lowbit_res = out.numpy()
+    log(f"[bench] low-bit two-stage, batched: {qps:.0f} queries/s ({ms_step:.3f} ms per {B}-query step)")
+    res = out.numpy()
 
-    # ---- FP32 CSR Top-K on the same GPU ----
-    for _ in range(2):
-        fp32_step()
-    stream.synchronize()
-    fsteps = max(2, a.steps // 4)
-    t0 = time.perf_counter()
-    for _ in range(fsteps):
-        fp32_step()
-    stream.synchronize()
-    fqps = fsteps * B / (time.perf_counter() - t0)
-    log(f"[bench] fp32 CSR Top-K: {fqps:.0f} queries/s; low-bit speedup {qps / fqps:.2f}x")
+    # ---- the same queries one at a time, and the plain FP32 CSR Top-K ----
+    s_steps = max(2, a.steps // 4)
+    single_qps = s_steps * B / timed(stream, single_step, s_steps, 1)
+    res1 = out1.numpy()
+    fqps = s_steps * B / timed(stream, fp32_step, s_steps, 1)
+    log(f"[bench] low-bit single-query {single_qps:.0f} q/s; fp32 CSR Top-K {fqps:.0f} q/s; "
+        f"batched/fp32 {qps / fqps:.2f}x")
+    same = bool(np.array_equal(res, res1))
 
-    # ---- roofline of the dominant kernel (stage 1) ----
-    reps = 50
-    s_ms, f_ms = [], []
-    for i in range(reps):
-        x = xs[i % B]
-        s_ms.append(event_time_ms(acc, stream, lambda: g.lowbit_stage1(x, CK, stream), 1))
-        f_ms.append(event_time_ms(acc, stream, lambda: g.lowbit_stage2(x, K, CK, outs[0], stream), 1))
-    stage1_ms, final_ms = float(np.mean(s_ms)), float(np.mean(f_ms))
+    # ---- per-launch times from HIP events on the query stream ----
+    g.set_profiling(True)
+    bt, lb, fp = [], [], []
+    for i in range(10):
+        batch_step()
+        bt.append(g.stage_times())
+    for i in range(40):
+        g.topk_lowbit(xs[i % B], K, CK, outs1[0], stream)
+        lb.append(g.stage_times())
+        g.topk_fp32(xs[i % B], K, outs1[0], stream)
+        fp.append(g.stage_times())
+    g.set_profiling(False)
+    bt_ms, lb_ms, fp_ms = np.mean(bt, axis=0), np.mean(lb, axis=0), np.mean(fp, axis=0)
+    kept = pk.info.nnz_kept
+    bs_tops = 2.0 * kept * B / (bt_ms[0] * 1e-3) / 1e12
+    bs_bytes = pk.info.stream_bytes + B * a.cols * 4 + B * pk.info.nslices * 64 * 4
     s1_bytes = pk.info.stream_bytes + a.cols * 4
-    achieved = s1_bytes / (stage1_ms * 1e-3) / 1e9
-    fp32_ms = event_time_ms(acc, stream, lambda: g.topk_fp32(xs[0], K, outs[0], stream), 20)
-    log(f"[bench] stage1 {stage1_ms * 1e3:.1f}us ({achieved:.0f} GB/s algorithmic), stage2 {final_ms * 1e3:.1f}us, "
-        f"fp32 query {fp32_ms * 1e3:.1f}us")
+    s1_gbs = s1_bytes / (lb_ms[0] * 1e-3) / 1e9
+    fp32_spmv_gbs = g.info.csr_bytes / (fp_ms[0] * 1e-3) / 1e9
+    log(f"[bench] batch: scores {bt_ms[0] * 1e3:.1f}us ({bs_tops:.1f} TOP/s int8, "
+        f"{bs_bytes / (bt_ms[0] * 1e-3) / 1e9:.0f} GB/s), select {bt_ms[1] * 1e3:.1f}us, final {bt_ms[2] * 1e3:.1f}us")
+    log(f"[bench] single: stage1 {lb_ms[0] * 1e3:.1f}us ({s1_gbs:.0f} GB/s), merge {lb_ms[1] * 1e3:.1f}us, "
+        f"final {lb_ms[2] * 1e3:.1f}us; fp32 spmv {fp_ms[0] * 1e3:.1f}us ({fp32_spmv_gbs:.0f} GB/s), "
+        f"select {fp_ms[1] * 1e3:.1f}us + {fp_ms[2] * 1e3:.1f}us")
 
     # ---- accuracy vs FP64 exact Top-K ----
     y64 = acc.DeviceBuffer((a.rows,), np.float64)
-    prec = []
-    allfound = []
+    prec, allfound = [], []
     for i in range(B):
         g.spmv_fp64(xs[i], y64, stream)
         stream.synchronize()
         yy = y64.numpy()
         top = np.argpartition(-yy, K)[:K + 1]
         exact = top[np.lexsort((top, -yy[top]))][:K]
-        got = lowbit_res[i, :, 1]
+        got = res[i, :, 1]
         p = [len(set(got[:k]) & set(exact[:k])) / k for k in KS]
         prec.append(p)
         allfound.append([float(x == 1.0) for x in p])
@@ -168,17 +175,16 @@ def main():
     recall_all = dict(zip(map(str, KS), np.round(np.mean(allfound, axis=0), 5).tolist()))
     log(f"[bench] precision@K {precision}")
 
-    # ---- traffic from rocprof counters, if profiled for this workload ----
+    # ---- HBM traffic from rocprof counters, when profiled for this workload ----
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "lowbit_stage1_400000x500_8bit_pmc.json")
+    pmc = os.path.join(ROOT, "profiles", "batch_scores_400000x500_8bit_pmc.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    # ---- CPU baseline ----
     cpu = None
     if not a.no_cpu:
-        cq, n = cpu_baseline(acc, csr, X, K, a.cpu_seconds)
+        cq, n = cpu_baseline(csr, X, K, a.cpu_seconds)
         cpu = {"value": round(cq, 3), "unit": "queries/s", "cores": 1, "kind": "port",
                "sample": f"{n} queries of the same workload through the reference CPU path semantics "
                          f"(FP32 CSR SpMV + min-heap Top-{K}, cpu_mkl_baseline.cpp:249-275) restated in "
@@ -199,28 +205,38 @@ def main():
         "dtype": "u8",
         "data": "synthetic",
         "config": {
-            "workload": f"low-bit two-stage Top-{K}: {a.rows}x{a.cols}, row nnz U[{a.deg[0]},{a.deg[1]}], "
+            "workload": f"low-bit two-stage Top-{K}, batched: {a.rows}x{a.cols}, row nnz U[{a.deg[0]},{a.deg[1]}], "
                         f"|N(0,1)| L2-normalised, {a.bits}-bit packets, SPLIT={a.split}, CK={CK}, "
                         f"{B} N(0,1) queries per step",
             "rows": a.rows, "cols": a.cols, "nnz": csr.nnz, "bits": a.bits, "split": a.split, "ck": CK,
             "k": K, "queries_per_step": B, "parallelism": "single GPU",
         },
         "roofline": {
-            "bound": "hbm",
-            "kernel": "lowbit_stage1",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "bound": "mfma",
+            "kernel": "batch_scores",
+            "achieved": round(bs_tops, 2),
+            "peak": INT8_PEAK_TOPS,
+            "unit": "TFLOP/s",
+            "frac": round(bs_tops / INT8_PEAK_TOPS, 5),
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": s1_bytes,
-            "avg_launch_us": round(stage1_ms * 1e3, 2),
+            "algorithmic_ops_per_launch": 2 * kept * B,
+            "avg_launch_us": round(float(bt_ms[0]) * 1e3, 2),
+            "note": "int8 MACs on VALU dot4 (one useful byte lane); peak is the dense int8 matrix-core figure",
+        },
+        "roofline_single_query_stage1": {
+            "bound": "hbm", "kernel": "lowbit_stage1", "achieved": round(s1_gbs, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(s1_gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": s1_bytes,
+            "avg_launch_us": round(float(lb_ms[0]) * 1e3, 2),
         },
         "cpu_baseline": cpu,
+        "single_query_qps": round(single_qps, 2),
         "fp32_csr_gpu_qps": round(fqps, 2),
-        "lowbit_vs_fp32_speedup": round(qps / fqps, 3),
-        "stage2_us": round(final_ms * 1e3, 2),
-        "fp32_query_us": round(fp32_ms * 1e3, 2),
+        "lowbit_batched_vs_fp32_speedup": round(qps / fqps, 3),
+        "batched_equals_single": same,
+        "batch_launch_us": [round(float(v) * 1e3, 2) for v in bt_ms],
+        "lowbit_launch_us": [round(float(v) * 1e3, 2) for v in lb_ms],
+        "fp32_launch_us": [round(float(v) * 1e3, 2) for v in fp_ms],
+        "fp32_spmv_gbs": round(fp32_spmv_gbs, 1),
         "bytes_per_nnz_packed": round(pk.info.bytes_per_nnz, 4),
         "bytes_per_nnz_fp32": round(g.info.csr_bytes / csr.nnz, 4),
         "precision_at_k": precision,

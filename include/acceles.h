@@ -217,6 +217,16 @@ typedef struct {
 } acc_gpu_info_t;
 
 int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info);
+
+/* Per-launch timing with profiling events (CL_QUEUE_PROFILING_ENABLE and
+ * get_event_execution_time, opencl_utils.hpp:294,335-344; the per-phase
+ * clocks of host_spmv_ucsr.cpp:569-572).  While enabled, every query records
+ * HIP events around each of its launches on the query's stream;
+ * acc_gpu_stage_times waits for the last query and returns up to n launch
+ * durations in ms (low-bit: stage 1, merge+rescore, final Top-K; FP32:
+ * SpMV, select, final select) and their count. */
+int acc_gpu_set_profiling(acc_gpu_matrix *m, int enable);
+int acc_gpu_stage_times(acc_gpu_matrix *m, float *ms, int n, int *count);
 void acc_gpu_matrix_destroy(acc_gpu_matrix *m);
 
 /* Plain FP32 CSR Top-K: SpMV (cpu_mkl_baseline.cpp:261) fused with the Top-k
@@ -234,15 +244,6 @@ int acc_gpu_topk_fp32(acc_gpu_matrix *m, const float *x_dev, int k, acc_element 
  * k <= ck, ck/split <= 64. */
 int acc_gpu_topk_lowbit(acc_gpu_matrix *m, const float *x_dev, int k, int ck,
                         acc_element *out_dev, void *stream);
-
-/* The two launches of acc_gpu_topk_lowbit, separately: stage 1 (INT32 SpMV
- * and per-partition select into the handle's candidate lists; the FPGA
- * kernel launch of host_spmv_ucsr.cpp:556-565) and stage 2 (merge, FP32
- * rescoring, Top-K; Q_read_result :691-748 plus quantify_util.py:354-367).
- * Stage 2 must follow stage 1 on the same stream with the same x and ck. */
-int acc_gpu_lowbit_stage1(acc_gpu_matrix *m, const float *x_dev, int ck, void *stream);
-int acc_gpu_lowbit_stage2(acc_gpu_matrix *m, const float *x_dev, int k, int ck, acc_element *out_dev,
-                          void *stream);
 
 /* FP64 exact scores y_dev[rows] = A x with FP64 accumulation (the exact
  * reference of quantify_util.py:257-258), for precision/recall reporting. */

@@ -165,14 +165,14 @@ def test_lowbit_two_stage_bitexact(acc, gpu, rows, cols, deg, bits, split, signe
         np.testing.assert_array_equal(v, ov)
 
 
-def test_lowbit_stages_and_host_form_agree(acc, gpu):
+def test_lowbit_stream_and_host_form_agree(acc, gpu):
     csr, pk, g = make(acc, 50000, 500, (40, 100), 8, 32, seed=5)
     x = acc.generate_queries(1, 500, seed=5)[0]
     xd = dev(acc, x)
     out = empty(acc, 200, np.int32)
-    g.lowbit_stage1(xd, 512)
-    g.lowbit_stage2(xd, 100, 512, out)
-    acc.synchronize()
+    st = acc.Stream()
+    g.topk_lowbit(xd, 100, 512, out, st)
+    st.synchronize()
     v, i = els(out)
     h = g.topk_lowbit_host(x, 100, 512)
     np.testing.assert_array_equal(h["index"], i)
@@ -193,6 +193,29 @@ def test_lowbit_batch_uneven_k(acc, gpu):
         _, oi, ov = O.two_stage(ptr, idx, val, q, X[qi], 900, g.info.group, 32, 512, int(ks[qi]))
         np.testing.assert_array_equal(a[qi, :ks[qi], 1], oi)
         np.testing.assert_array_equal(a[qi, :ks[qi], 0].view(np.float32), ov)
+
+
+@pytest.mark.parametrize("rows,cols,deg,bits,split,signed,nq", [
+    (20000, 500, (40, 100), 8, 32, False, 40),   # unsigned 8-bit: offset-byte dot4, 32-query blocks
+    (20000, 900, (30, 90), 8, 16, True, 37),     # signed 8-bit
+    (20000, 500, (40, 100), 4, 32, False, 9),    # 4-bit, 16-query block
+    (30000, 1024, (0.5, 40.0), 8, 64, False, 33),  # power-law rows
+])
+def test_lowbit_batch_matches_oracle(acc, gpu, rows, cols, deg, bits, split, signed, nq):
+    kind = 1 if isinstance(deg[0], float) else 0
+    csr, pk, g = make(acc, rows, cols, deg, bits, split, signed, kind, seed=rows + nq)
+    ptr, idx, val, q = oracle_qv(csr, bits, signed)
+    X = acc.generate_queries(nq, cols, seed=nq)
+    ck = min(64, 512 // split) * split
+    ks = np.array([KS[i % len(KS)] for i in range(nq)], np.int32)
+    out = empty(acc, nq * 100 * 2, np.int32)
+    g.topk_lowbit_batch(dev(acc, X), ks, ck, out, 100)
+    acc.synchronize()
+    a = out.numpy().reshape(nq, 100, 2)
+    for qi in range(nq):
+        _, oi, ov = O.two_stage(ptr, idx, val, q, X[qi], cols, g.info.group, split, ck, int(ks[qi]))
+        np.testing.assert_array_equal(a[qi, :ks[qi], 1], oi, err_msg=f"query {qi}")
+        np.testing.assert_array_equal(a[qi, :ks[qi], 0].view(np.float32), ov, err_msg=f"query {qi}")
 
 
 def test_fp64_exact_scores(acc, gpu):
