@@ -18,14 +18,27 @@ __global__ __launch_bounds__(ACC_WG) void batch_select(BatchArgs a) {
     const int32_t *Sq = a.S + (size_t)q * a.nslots;
     WaveTopM<2> col;
     col.init(colbuf + wave * WaveTopM<2>::cap, a.M);
-    for (int64_t i0 = w0; i0 < w1; i0 += 64) {
-        const int64_t slot = i0 + lane;
-        const int32_t sc = __builtin_nontemporal_load(Sq + slot);
-        const bool maybe = sc >= dkey_score_i32(col.thr);
-        if (__ballot(maybe) == 0) continue;
-        const int32_t row = maybe ? a.perm[slot] : -1;
-        const uint64_t key = dkey_i32(sc, row);
-        col.push(maybe && row >= 0, key);
+    // U scores per lane per iteration; the rows of the slots that pass the
+    // current threshold are fetched together, one latency per 64*U slots
+    constexpr int U = 8;
+    for (int64_t i0 = w0; i0 < w1; i0 += 64 * U) {
+        int32_t sc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)  // w1 - i0 is a multiple of 64: the guard is wave-uniform
+            sc[u] = i0 + u * 64 < w1 ? __builtin_nontemporal_load(Sq + i0 + u * 64 + lane) : INT32_MIN;
+        const int32_t ts = dkey_score_i32(col.thr);
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) any |= i0 + u * 64 < w1 && sc[u] >= ts;
+        if (__ballot(any) == 0) continue;
+        int32_t row[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool maybe = i0 + u * 64 < w1 && sc[u] >= ts;
+            row[u] = maybe ? a.perm[i0 + u * 64 + lane] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) col.push(row[u] >= 0, dkey_i32(sc[u], row[u]));
     }
     col.compact();
     if (lane == 0) s_n[wave] = col.n;
@@ -69,11 +82,7 @@ __global__ __launch_bounds__(ACC_WG) void batch_final(BatchArgs a) {
     const uint64_t *keys = a.resc + (size_t)q * total;
     WaveTopM<R> col;
     col.init(colbuf + wave * WaveTopM<R>::cap, K);
-    for (int b = wave * 64; b < total; b += ACC_WG) {
-        const int i = b + lane;
-        const uint64_t k = i < total ? keys[i] : 0ull;
-        col.push(k != 0ull, k);
-    }
+    push_range(col, keys, (uint32_t)total, (uint32_t)wave * 64, ACC_WG);
     col.compact();
     if (lane == 0) s_n[wave] = col.n;
     __syncthreads();

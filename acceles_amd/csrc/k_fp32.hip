@@ -75,15 +75,25 @@ __global__ __launch_bounds__(ACC_WG) void select_blocks(SelectArgs a) {
     const int64_t w0 = b0 + wave * q, w1 = min(b1, w0 + q);
     WaveTopM<R> col;
     col.init(colbuf + wave * WaveTopM<R>::cap, a.K);
-    uint64_t g = atomic_load_u64(a.thr);
-    int step = 0;
-    for (int64_t i0 = w0; i0 < w1; i0 += 64) {
-        const int64_t i = i0 + lane;
-        const bool valid = i < w1;
-        const uint64_t key = valid ? dkey_f32(__builtin_nontemporal_load(a.y + i), (int32_t)i) : 0ull;
-        const uint64_t t = col.thr > g ? col.thr : g;
-        col.push(valid && key > t, key);
-        if ((++step & 15) == 0) g = atomic_load_u64(a.thr);
+    // U scores per lane are loaded together (with the shared threshold) so one
+    // memory latency covers 64*U keys
+    constexpr int U = 8;
+    for (int64_t i0 = w0; i0 < w1; i0 += 64 * U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + u * 64 + lane;
+            v[u] = i < w1 ? __builtin_nontemporal_load(a.y + i) : 0.f;
+        }
+        const uint64_t g = atomic_load_u64(a.thr);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + u * 64 + lane;
+            const bool valid = i < w1;
+            const uint64_t key = valid ? dkey_f32(v[u], (int32_t)i) : 0ull;
+            const uint64_t t = col.thr > g ? col.thr : g;
+            col.push(valid && key > t, key);
+        }
     }
     col.compact();
     if (lane == 0) s_n[wave] = col.n;
@@ -105,10 +115,7 @@ __global__ __launch_bounds__(ACC_WG) void select_final(SelectArgs a) {
     const uint32_t n = *a.cnt;
     WaveTopM<R> col;
     col.init(colbuf + wave * WaveTopM<R>::cap, a.K);
-    for (uint32_t b = wave * 64; b < n; b += ACC_WG) {
-        const uint32_t i = b + lane;
-        col.push(i < n, i < n ? a.cand[i] : 0ull);
-    }
+    push_range(col, a.cand, n, (uint32_t)wave * 64, ACC_WG);
     col.compact();
     if (lane == 0) s_n[wave] = col.n;
     __syncthreads();

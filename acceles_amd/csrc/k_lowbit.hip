@@ -95,10 +95,7 @@ __global__ __launch_bounds__(LB_WG) void lowbit_merge(LowbitArgs a) {
     const uint64_t *keys = a.cand + a.cand_off[p];
     WaveTopM<2> col;
     col.init(colbuf + wave * WaveTopM<2>::cap, a.M);
-    for (uint32_t b = wave * 64; b < n; b += LB_WG) {
-        const uint32_t i = b + lane;
-        col.push(i < n, i < n ? keys[i] : 0ull);
-    }
+    push_range(col, keys, n, (uint32_t)wave * 64, LB_WG);
     col.compact();
     if (lane == 0) s_n[wave] = col.n;
     __syncthreads();
@@ -144,11 +141,7 @@ __global__ __launch_bounds__(ACC_WG) void lowbit_final(LowbitArgs a) {
     const int total = a.split * a.M;
     WaveTopM<R> col;
     col.init(colbuf + wave * WaveTopM<R>::cap, a.K);
-    for (int b = wave * 64; b < total; b += ACC_WG) {
-        const int i = b + lane;
-        const uint64_t k = i < total ? a.resc[i] : 0ull;
-        col.push(k != 0ull, k);
-    }
+    push_range(col, a.resc, (uint32_t)total, (uint32_t)wave * 64, ACC_WG);
     col.compact();
     if (lane == 0) s_n[wave] = col.n;
     __syncthreads();

@@ -103,6 +103,26 @@ __device__ inline void wg_merge(WaveTopM<R> &col, const uint64_t *colbuf, const 
     col.compact();
 }
 
+// Offers keys[i] for i = first + lane, first + stride + lane, ... (< n) to the
+// collector; U loads per lane are issued together so one memory latency
+// covers 64*U keys.  Zero keys (empty slots) are skipped.  stride is a
+// multiple of 64 (the workgroup size when each wave starts at wave*64).
+template <int R, int U = 8>
+__device__ inline void push_range(WaveTopM<R> &col, const uint64_t *keys, uint32_t n, uint32_t first,
+                                  uint32_t stride) {
+    const uint32_t lane = (uint32_t)lane_id();
+    for (uint32_t b = first; b < n; b += stride * U) {
+        uint64_t k[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = b + u * stride + lane;
+            k[u] = i < n ? keys[i] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) col.push(k[u] != 0ull, k[u]);
+    }
+}
+
 // Wave 0: publish the list's M-th key as a lower bound, then append the keys
 // that can still be in the final Top-M to the shared candidate list.
 template <int R>
