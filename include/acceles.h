@@ -231,7 +231,7 @@ void acc_gpu_matrix_destroy(acc_gpu_matrix *m);
 
 /* Plain FP32 CSR Top-K: SpMV (cpu_mkl_baseline.cpp:261) fused with the Top-k
  * of top_k_heap (:128-149); the GPU baseline spec is
- * host_spmv_topk_csr_gpu.cu:171-231.  k <= 1024.  x_dev: cols floats;
+ * host_spmv_topk_csr_gpu.cu:171-231.  1 <= k <= 960.  x_dev: cols floats;
  * out_dev: k elements in ranking order (tail (-inf, -1) if k > rows). */
 int acc_gpu_topk_fp32(acc_gpu_matrix *m, const float *x_dev, int k, acc_element *out_dev,
                       void *stream);
