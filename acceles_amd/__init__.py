@@ -29,6 +29,7 @@ ERRORS = {
 }
 DEG_UNIFORM, DEG_GAMMA = 0, 1
 VAL_ABSNORMAL, VAL_NORMAL, VAL_UNIFORM = 0, 1, 2
+BATCH_AUTO, BATCH_SPARSE, BATCH_DENSE = 0, 1, 2
 
 
 class AccelesError(RuntimeError):
@@ -61,7 +62,8 @@ class GpuInfo(C.Structure):
     _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("split", C.c_int32), ("bits", C.c_int32),
                 ("has_csr", C.c_int32), ("has_packed", C.c_int32), ("group", C.c_int32),
                 ("work_items", C.c_int32), ("nnz", C.c_int64), ("csr_bytes", C.c_int64),
-                ("stream_bytes", C.c_int64), ("device_bytes", C.c_int64)]
+                ("stream_bytes", C.c_int64), ("device_bytes", C.c_int64),
+                ("batch_mode", C.c_int32), ("dense_ok", C.c_int32), ("dense_bytes", C.c_int64)]
 
 
 _P = C.c_void_p
@@ -117,6 +119,7 @@ SIGNATURES = [
     ("acc_gpu_matrix_info", C.c_int, [_P, C.POINTER(GpuInfo)]),
     ("acc_gpu_matrix_destroy", None, [_P]),
     ("acc_gpu_set_profiling", C.c_int, [_P, C.c_int]),
+    ("acc_gpu_set_batch_mode", C.c_int, [_P, C.c_int, C.c_int]),
     ("acc_gpu_stage_times", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
     ("acc_gpu_topk_fp32", C.c_int, [_P, _P, C.c_int, _P, _P]),
     ("acc_gpu_topk_lowbit", C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P]),
@@ -479,9 +482,19 @@ class GpuMatrix:
         _check(lib().acc_gpu_upload(csr._h if csr is not None else None,
                                     packed._h if packed is not None else None, device, C.byref(h)))
         self._h = h
-        self.info = GpuInfo()
-        _check(lib().acc_gpu_matrix_info(self._h, C.byref(self.info)))
-        self.rows, self.cols = self.info.rows, self.info.cols
+        info = self.info
+        self.rows, self.cols = info.rows, info.cols
+
+    @property
+    def info(self):
+        """acc_gpu_info_t, read afresh (batch mode and dense tiles change after upload)."""
+        info = GpuInfo()
+        _check(lib().acc_gpu_matrix_info(self._h, C.byref(info)))
+        return info
+
+    def set_batch_mode(self, mode, min_dense_batch=0):
+        """BATCH_AUTO, BATCH_SPARSE or BATCH_DENSE scoring for topk_lowbit_batch."""
+        _check(lib().acc_gpu_set_batch_mode(self._h, int(mode), int(min_dense_batch)))
 
     def set_profiling(self, enable=True):
         _check(lib().acc_gpu_set_profiling(self._h, int(enable)))

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 using acc::fail;
@@ -23,6 +24,7 @@ using acc::fail;
 static constexpr int kMaxM = 64;     // rows kept per partition
 static constexpr int kMaxK = 960;    // Top-K limit of the FP32 path (collector cap 1024)
 static constexpr int kMaxCK = 4096;  // candidates of the low-bit path
+static constexpr size_t kDenseLds = 160 * 1024 - 1024;  // LDS of a dense_scores workgroup
 
 struct acc_gpu_matrix {
     int device = 0;
@@ -64,9 +66,29 @@ struct acc_gpu_matrix {
     size_t bresc_elems = 0;
     int32_t *d_bK = nullptr;
     int bK_elems = 0;
+    // dense int8 tiles of the batched path (matrix cores), built on demand
+    uint8_t *d_nvalid = nullptr;     // valid rows per slice
+    bool dense_ok = false;           // the layout can hold this matrix exactly
+    std::string dense_why;           // why not, otherwise
+    bool has_dense = false;
+    int batch_mode = ACC_BATCH_AUTO;
+    int dense_min_nq = 16;           // ACC_BATCH_AUTO: dense tiles from this batch size on
+    int cols_pad = 0, nkb = 0, u8off = 0;
+    int8_t *d_tiles = nullptr;
+    int64_t dense_bytes = 0;
+    int8_t *d_qtab = nullptr;
+    size_t qtab_elems = 0;
+    int8_t *d_qplain = nullptr;
+    size_t qplain_elems = 0;
+    int32_t *d_corr = nullptr;
+    size_t corr_elems = 0;
+    uint32_t *d_tk = nullptr;
+    size_t tk_elems = 0;
+    uint64_t *d_ckeys = nullptr;
+    size_t ckeys_elems = 0;
     // profiling events around the launches of the last query
     bool profiling = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int nev = 0;  // events recorded by the last query
     // host-call staging
     float *d_xstage = nullptr;
@@ -275,6 +297,144 @@ int launch_batch(acc_gpu_matrix *m, const float *X, int nq, int M, int kmax, acc
     if (!ff) return fail(ACC_ERR_ARG, "k too large");
     hipLaunchKernelGGL(ff, dim3(nq), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
     mark(m, 3, st);
+    return check_launch();
+}
+
+// Raises a kernel's dynamic LDS limit when it needs more than 64 KB.
+int lds_limit(const void *fn, size_t bytes) {
+    if (bytes <= 65536) return ACC_OK;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return fail(ACC_ERR_DEVICE, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+    return ACC_OK;
+}
+
+DenseArgs dense_args(const acc_gpu_matrix *m) {
+    DenseArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.words = m->d_words;
+    a.slices = m->d_slices;
+    a.perm = m->d_perm;
+    a.part_slice = m->d_part_slice;
+    a.nvalid = m->d_nvalid;
+    a.tiles = m->d_tiles;
+    a.nslices = m->nslices;
+    a.cols = m->cols;
+    a.cols_pad = m->cols_pad;
+    a.nkb = m->nkb;
+    a.col_bits = m->col_bits;
+    a.val_bits = m->bits;
+    a.is_signed = m->is_signed;
+    a.u8off = m->u8off;
+    a.split = m->split;
+    a.ptr = m->d_ptr;
+    a.idx = m->d_idx;
+    a.val = m->d_val;
+    a.G = m->group;
+    return a;
+}
+
+// Dense int8 tiles from the sliced packets on the device (dense_build).
+int build_dense(acc_gpu_matrix *m) {
+    if (m->has_dense) return ACC_OK;
+    if (!m->dense_ok) return fail(ACC_ERR_ARG, "dense tiles unavailable: " + m->dense_why);
+    const size_t bytes = (size_t)m->nslices * 2 * m->nkb * 1024;
+    int rc = dalloc(m, &m->d_tiles, bytes);
+    if (rc != ACC_OK) return rc;
+    DenseArgs a = dense_args(m);
+    const size_t lds = (size_t)m->nkb * 1024;
+    rc = lds_limit((const void *)get_dense_build(), lds);
+    if (rc != ACC_OK) return rc;
+    hipLaunchKernelGGL(get_dense_build(), dim3((unsigned)m->nslices), dim3(64), lds, 0, a);
+    rc = check_launch();
+    if (rc != ACC_OK) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    m->dense_bytes = (int64_t)bytes;
+    m->has_dense = true;
+    return ACC_OK;
+}
+
+bool use_dense(const acc_gpu_matrix *m, int nq) {
+    if (m->batch_mode == ACC_BATCH_DENSE) return true;
+    return m->batch_mode == ACC_BATCH_AUTO && m->dense_ok && nq >= m->dense_min_nq;
+}
+
+// Batched low-bit queries on the dense tiles: dense_prep, dense_scores,
+// dense_select, then (out != null) batch_rescore and batch_topk.
+int launch_dense(acc_gpu_matrix *m, const float *X, int nq, int M, int kmax, acc_element *out, int out_stride,
+                 int32_t *crow, int32_t *cscore, hipStream_t st) {
+    int rc = build_dense(m);
+    if (rc != ACC_OK) return rc;
+    int NT = nq <= 32 ? 1 : (nq <= 64 ? 2 : 4);
+    while (NT > 1 && (size_t)m->nkb * NT * 1024 + 128 * NT > kDenseLds) NT >>= 1;
+    const int QB = 32 * NT;
+    const int nqb = (nq + QB - 1) / QB;
+    const size_t cand = (size_t)nq * m->split * M;
+    rc = grow(m, &m->d_qtab, &m->qtab_elems, (size_t)nqb * m->nkb * NT * 1024);
+    if (rc == ACC_OK) rc = grow(m, &m->d_qplain, &m->qplain_elems, (size_t)nq * m->cols_pad);
+    if (rc == ACC_OK) rc = grow(m, &m->d_corr, &m->corr_elems, (size_t)nqb * QB);
+    if (rc == ACC_OK) rc = grow(m, &m->d_tk, &m->tk_elems, (size_t)nq * m->nslices * 4);
+    if (rc == ACC_OK) rc = grow(m, &m->d_ckeys, &m->ckeys_elems, cand);
+    if (rc == ACC_OK) rc = grow(m, &m->d_bresc, &m->bresc_elems, cand);
+    if (rc != ACC_OK) return rc;
+    DenseArgs a = dense_args(m);
+    a.X = X;
+    a.nq = nq;
+    a.NT = NT;
+    a.nqb = nqb;
+    a.qtab = m->d_qtab;
+    a.qplain = m->d_qplain;
+    a.corr = m->d_corr;
+    a.tk = m->d_tk;
+    a.M = M;
+    a.ckeys = m->d_ckeys;
+    a.resc = m->d_bresc;
+    a.K = m->d_bK;
+    a.out = out;
+    a.out_stride = out_stride;
+    a.cand_rows_out = crow;
+    a.cand_scores_out = cscore;
+
+    mark(m, 0, st);
+    hipLaunchKernelGGL(get_dense_prep(), dim3(nqb * QB), dim3(ACC_WG), 0, st, a);
+    if ((rc = check_launch()) != ACC_OK) return rc;
+    mark(m, 1, st);
+    dense_fn fs = get_dense_scores(NT);
+    const size_t lds_s = (size_t)m->nkb * NT * 1024 + (size_t)QB * 4;
+    if ((rc = lds_limit((const void *)fs, lds_s)) != ACC_OK) return rc;
+    const int64_t groups = (m->nslices + 7) / 8;
+    const int nwx = (int)std::max<int64_t>(1, std::min<int64_t>(groups, std::max(1, m->cus / nqb)));
+    hipLaunchKernelGGL(fs, dim3(nwx * nqb), dim3(512), lds_s, st, a);
+    if ((rc = check_launch()) != ACC_OK) return rc;
+    mark(m, 2, st);
+    const size_t lds_sel = (size_t)ACC_WAVES * 256 * 8 + (size_t)ACC_WAVES * m->cols_pad;
+    if ((rc = lds_limit((const void *)get_dense_select(), lds_sel)) != ACC_OK) return rc;
+    hipLaunchKernelGGL(get_dense_select(), dim3(m->split, (nq + ACC_WAVES - 1) / ACC_WAVES), dim3(ACC_WG), lds_sel, st, a);
+    if ((rc = check_launch()) != ACC_OK) return rc;
+    mark(m, 3, st);
+    if (!out) return ACC_OK;
+    const int64_t threads = (int64_t)cand * m->group;
+    hipLaunchKernelGGL(get_batch_rescore(), dim3((unsigned)((threads + ACC_WG - 1) / ACC_WG)), dim3(ACC_WG), 0, st, a);
+    if ((rc = check_launch()) != ACC_OK) return rc;
+    mark(m, 4, st);
+    const int total = m->split * M;
+    if (total <= 1024) {
+        dense_fn ft = get_batch_topk(pow2_at_least((total + 63) / 64));
+        hipLaunchKernelGGL(ft, dim3((nq + ACC_WAVES - 1) / ACC_WAVES), dim3(ACC_WG), 0, st, a);
+    } else {
+        BatchArgs b;
+        std::memset(&b, 0, sizeof(b));
+        b.split = m->split;
+        b.M = M;
+        b.resc = m->d_bresc;
+        b.K = m->d_bK;
+        b.out = out;
+        b.out_stride = out_stride;
+        const int R = r_for(kmax);
+        batch_fn ff = get_batch_final(R);
+        if (!ff) return fail(ACC_ERR_ARG, "k too large");
+        hipLaunchKernelGGL(ff, dim3(nq), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, b);
+    }
+    mark(m, 5, st);
     return check_launch();
 }
 
@@ -503,6 +663,31 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         COPY(m->d_cand_off, coff.data(), coff.size() * 8);
         ZERO(m->d_thr, (size_t)m->split * 8);
         ZERO(m->d_cnt, (size_t)m->split * 4);
+        {
+            // valid rows per slice: padding lanes (perm -1) close a partition's last slice
+            std::vector<uint8_t> nv((size_t)m->nslices, 0);
+            for (int64_t sl2 = 0; sl2 < m->nslices; ++sl2) {
+                int c = 0;
+                while (c < 64 && pk->perm[(size_t)sl2 * 64 + c] >= 0) ++c;
+                nv[(size_t)sl2] = (uint8_t)c;
+            }
+            TRY(dalloc(m, &m->d_nvalid, nv.size()));
+            COPY(m->d_nvalid, nv.data(), nv.size());
+        }
+        m->cols_pad = (m->cols + 31) & ~31;
+        m->nkb = m->cols_pad / 32;
+        m->u8off = (!m->is_signed && m->bits == 8) ? 1 : 0;
+        {
+            const int64_t maxv = m->is_signed ? (1 << (m->bits - 1)) : ((1 << m->bits) - 1);
+            if (pk->dup_rows > 0)
+                m->dense_why = "a row names a column twice (the dense layout holds one value per column)";
+            else if (maxv * 128 * (int64_t)m->cols >= (int64_t(1) << 25))
+                m->dense_why = "scores may reach 2^25 (too many columns for the value width)";
+            else if ((size_t)m->nkb * 1024 + 128 > kDenseLds)
+                m->dense_why = "more than 4960 columns";
+            else
+                m->dense_ok = true;
+        }
     }
     TRY(dalloc(m, &m->d_xstage, (size_t)m->cols));
     TRY(dalloc(m, &m->d_ostage, (size_t)kMaxCK));
@@ -533,12 +718,32 @@ extern "C" int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info
     info->csr_bytes = m->has_csr ? m->nnz * 8 + ((int64_t)m->rows + 1) * 4 : 0;
     info->stream_bytes = m->stream_bytes;
     info->device_bytes = m->device_bytes;
+    info->batch_mode = m->batch_mode;
+    info->dense_ok = m->dense_ok ? 1 : 0;
+    info->dense_bytes = m->dense_bytes;
     return ACC_OK;
 }
 
 extern "C" void acc_gpu_matrix_destroy(acc_gpu_matrix *m) {
     free_all(m);
     delete m;
+}
+
+extern "C" int acc_gpu_set_batch_mode(acc_gpu_matrix *m, int mode, int min_dense_batch) {
+    if (!m) return fail(ACC_ERR_ARG, "null matrix");
+    if (mode != ACC_BATCH_AUTO && mode != ACC_BATCH_SPARSE && mode != ACC_BATCH_DENSE)
+        return fail(ACC_ERR_ARG, "mode must be ACC_BATCH_AUTO, ACC_BATCH_SPARSE or ACC_BATCH_DENSE");
+    if (min_dense_batch < 0) return fail(ACC_ERR_ARG, "min_dense_batch must be >= 0");
+    if (mode != ACC_BATCH_SPARSE && !m->has_packed) return fail(ACC_ERR_ARG, "matrix was uploaded without low-bit packets");
+    if (mode == ACC_BATCH_DENSE && !m->dense_ok) return fail(ACC_ERR_ARG, "dense tiles unavailable: " + m->dense_why);
+    HIP_TRY(hipSetDevice(m->device));
+    if (mode != ACC_BATCH_SPARSE && m->dense_ok) {
+        int rc = build_dense(m);
+        if (rc != ACC_OK) return rc;
+    }
+    m->batch_mode = mode;
+    if (min_dense_batch > 0) m->dense_min_nq = min_dense_batch;
+    return ACC_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -567,6 +772,15 @@ extern "C" int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, 
     if (rc != ACC_OK) return rc;
     HIP_TRY(hipSetDevice(m->device));
     hipStream_t st = (hipStream_t)stream;
+    const int M = ck / m->split;
+    if (use_dense(m, nq)) {
+        size_t kcap = (size_t)m->bK_elems;
+        rc = grow(m, &m->d_bK, &kcap, (size_t)nq);
+        m->bK_elems = (int)kcap;
+        if (rc != ACC_OK) return rc;
+        HIP_TRY(hipMemcpyAsync(m->d_bK, k_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+        return launch_dense(m, x_dev, nq, M, kmax, out_dev, out_stride, nullptr, nullptr, st);
+    }
     if ((size_t)m->cols * (16 + 8) > 65536) {
         // the query table of a block does not fit in LDS: one query per pass
         for (int q = 0; q < nq && rc == ACC_OK; ++q)
@@ -574,7 +788,6 @@ extern "C" int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, 
                                out_dev + (size_t)q * out_stride, nullptr, nullptr, st);
         return rc;
     }
-    const int M = ck / m->split;
     const size_t nslots = (size_t)m->nslices * 64;
     rc = grow(m, &m->d_bS, &m->bS_elems, (size_t)nq * nslots);
     if (rc == ACC_OK) rc = grow(m, &m->d_bresc, &m->bresc_elems, (size_t)nq * m->split * M);

@@ -31,6 +31,7 @@ struct acc_packed {
     int32_t bits = 8, col_bits = 10, entry_bits = 18, is_signed = 0;
     float scale = 0.f;
     int64_t nnz = 0, nnz_kept = 0;
+    int64_t dup_rows = 0;                   // rows holding a column twice among kept entries
     std::vector<int32_t> part_row_begin;    // split + 1
     std::vector<int64_t> part_slice_begin;  // split + 1
     std::vector<uint32_t> slice_off;        // in units of 64 words

@@ -260,6 +260,57 @@ struct QTable {
 
 
 // ---------------------------------------------------------------------------
+// batched low-bit path on dense int8 tiles (matrix cores)
+// ---------------------------------------------------------------------------
+//   dense_build    sliced packets -> dense int8 tiles, once per matrix;
+//   dense_prep     quantizes the batch's queries into the B-operand table;
+//   dense_scores   v_mfma_i32_32x32x32_i8 over each 64-row slice and a block
+//                  of 32*NT queries; per (query, slice) it keeps only the four
+//                  best (score, row-in-slice) keys;
+//   dense_select   per (partition, query): threshold = the M-th best slice
+//                  maximum, candidates from the four-key lists, exact rescan of
+//                  any slice whose list may have dropped a candidate, Top-M;
+//   batch_rescore  FP32 rescoring of every (query, candidate);
+//   batch_topk     per query: its Top-K[q] of the rescored keys.
+//
+// Tile layout: tile (rt, kb) of slice s (rows s*64 + rt*32 .. +31, columns
+// kb*32 .. +31) is 1 KB at ((s*2 + rt)*nkb + kb)*1024; lane l's 16 bytes hold
+// row l%32, columns kb*32 + (l/32)*16 + 0..15 -- the A-operand register of
+// the MFMA.  The query table uses the same lane mapping for the B operand
+// (query l%32 of a 32-query tile), so every k is paired identically in both.
+// Unsigned 8-bit values are stored as v-128 (absent entries as -128) and
+// 128 * sum(query) is added back per query; other widths are stored as is.
+
+struct DenseArgs {
+    const uint32_t *words;       // sliced packets (dense_build, exact rescans)
+    const uint2 *slices;
+    const int32_t *perm;
+    const int64_t *part_slice;   // split + 1
+    const uint8_t *nvalid;       // valid rows per slice (a prefix of its lanes)
+    int8_t *tiles;
+    int64_t nslices;
+    int cols, cols_pad, nkb, col_bits, val_bits, is_signed, u8off;
+    const float *X;              // nq x cols FP32 queries
+    int nq, NT, nqb;
+    int8_t *qtab;                // nqb x nkb x NT x 1024 B-operand bytes
+    int8_t *qplain;              // nq x cols_pad quantized queries
+    int32_t *corr;               // nqb*32*NT per-query offsets
+    uint32_t *tk;                // nq x nslices x 4 keys
+    int split, M;
+    uint64_t *ckeys;             // nq x split x M (score, row) keys, 0 = none
+    const uint32_t *ptr;         // FP32 CSR for rescoring
+    const int32_t *idx;
+    const float *val;
+    int G;
+    uint64_t *resc;              // nq x split*M rescored keys
+    const int32_t *K;            // per query
+    acc_element *out;
+    int out_stride;
+    int32_t *cand_rows_out;      // optional: nq x split*M stage-1 candidates
+    int32_t *cand_scores_out;
+};
+
+// ---------------------------------------------------------------------------
 // host-side getters of the kernel instantiations (one translation unit each)
 // ---------------------------------------------------------------------------
 typedef void (*lowbit_fn)(LowbitArgs);
@@ -283,3 +334,10 @@ batch_fn get_batch_scores16(int E, bool is_signed, int bits);  // k_batch16.hip
 batch_fn get_batch_scores32(int E, bool is_signed, int bits);  // k_batch32.hip
 batch_fn get_batch_select();                                   // k_batch.hip
 batch_fn get_batch_final(int R);
+typedef void (*dense_fn)(DenseArgs);
+dense_fn get_dense_build();                                    // k_dense.hip
+dense_fn get_dense_prep();
+dense_fn get_dense_scores(int NT);
+dense_fn get_dense_select();
+dense_fn get_batch_rescore();
+dense_fn get_batch_topk(int R);

@@ -68,6 +68,29 @@ extern "C" int acc_pack_lowbit(const acc_csr *m, const acc_pack_opts *o, acc_pac
     }
     pk->nnz_kept = nkept;
 
+    // rows that name a column twice (the reference generators sample columns
+    // with replacement, create_matrices.py:52-54); a dense layout cannot hold them
+    int64_t dups = 0;
+#pragma omp parallel reduction(+ : dups)
+    {
+        std::vector<uint32_t> seen((size_t)(m->cols + 31) / 32 + 1, 0u);
+#pragma omp for schedule(static, 4096)
+        for (int32_t r = 0; r < R; ++r) {
+            bool dup = false;
+            const int64_t b = m->ptr[(size_t)r], e = m->ptr[(size_t)r + 1];
+            for (int64_t j = b; j < e; ++j) {
+                if (q[(size_t)j] == 0) continue;
+                const uint32_t c = (uint32_t)m->idx[(size_t)j];
+                uint32_t &w = seen[c >> 5];
+                if (w & (1u << (c & 31))) dup = true;
+                w |= 1u << (c & 31);
+            }
+            for (int64_t j = b; j < e; ++j) seen[(uint32_t)m->idx[(size_t)j] >> 5] = 0u;
+            dups += dup;
+        }
+    }
+    pk->dup_rows = dups;
+
     // partitions (quantify_util.py:185-189): rows/split each, remainder to the last
     const int S = o->split;
     const int32_t psize = R / S;

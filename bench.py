@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-mode", choices=["auto", "packets", "dense"], default="auto",
+                    help="scoring layout of the batched path")
     return ap.parse_args()
 
 
@@ -93,6 +95,7 @@ def main():
     csr = acc.CSRMatrix.generate(a.rows, a.cols, deg=tuple(a.deg), seed=0xACCE1E5 + 1)
     pk = csr.pack(bits=a.bits, split=a.split)
     g = acc.GpuMatrix(csr, pk, device=0)
+    g.set_batch_mode({"auto": acc.BATCH_AUTO, "packets": acc.BATCH_SPARSE, "dense": acc.BATCH_DENSE}[a.batch_mode])
     X = acc.generate_queries(a.batch, a.cols, seed=0xACCE1E5 + 2)
     Xd = acc.DeviceBuffer.from_numpy(X)
     log(f"[bench] setup {time.perf_counter() - t0:.1f}s: nnz={csr.nnz} packed={pk.info.stream_bytes / 1e6:.1f}MB "
@@ -124,6 +127,15 @@ def main():
     log(f"[bench] low-bit two-stage, batched: {qps:.0f} queries/s ({ms_step:.3f} ms per {B}-query step)")
     res = out.numpy()
 
+    # ---- the other batched layout, for comparison ----
+    other_qps = None
+    if g.info.dense_ok:
+        mode = g.info.batch_mode
+        g.set_batch_mode(acc.BATCH_SPARSE if mode != acc.BATCH_SPARSE else acc.BATCH_DENSE)
+        other_qps = a.steps * B / timed(stream, batch_step, a.steps, 1)
+        g.set_batch_mode(mode)
+        log(f"[bench] batched, other layout: {other_qps:.0f} queries/s")
+
     # ---- the same queries one at a time, and the plain FP32 CSR Top-K ----
     s_steps = max(2, a.steps // 4)
     single_qps = s_steps * B / timed(stream, single_step, s_steps, 1)
@@ -147,13 +159,29 @@ def main():
     g.set_profiling(False)
     bt_ms, lb_ms, fp_ms = np.mean(bt, axis=0), np.mean(lb, axis=0), np.mean(fp, axis=0)
     kept = pk.info.nnz_kept
-    bs_tops = 2.0 * kept * B / (bt_ms[0] * 1e-3) / 1e12
-    bs_bytes = pk.info.stream_bytes + B * a.cols * 4 + B * pk.info.nslices * 64 * 4
+    nslots = pk.info.nslices * 64
+    dense = len(bt_ms) == 5
+    if dense:
+        # dense int8 tiles on the matrix cores: prep, scores, select, rescore, final
+        names = ["prep", "scores", "select", "rescore", "final"]
+        t_sc = bt_ms[1]
+        cols_pad = (a.cols + 31) // 32 * 32
+        bs_bytes = g.info.dense_bytes + B * cols_pad + B * pk.info.nslices * 16
+        bs_ops = 2.0 * nslots * cols_pad * ((B + 31) // 32 * 32)
+        sc_kernel = "dense_scores"
+    else:
+        names = ["scores", "select", "final"]
+        t_sc = bt_ms[0]
+        bs_bytes = pk.info.stream_bytes + B * a.cols * 4 + B * nslots * 4
+        bs_ops = 2.0 * kept * B
+        sc_kernel = "batch_scores"
+    bs_gbs = bs_bytes / (t_sc * 1e-3) / 1e9
+    bs_tops = bs_ops / (t_sc * 1e-3) / 1e12
     s1_bytes = pk.info.stream_bytes + a.cols * 4
     s1_gbs = s1_bytes / (lb_ms[0] * 1e-3) / 1e9
     fp32_spmv_gbs = g.info.csr_bytes / (fp_ms[0] * 1e-3) / 1e9
-    log(f"[bench] batch: scores {bt_ms[0] * 1e3:.1f}us ({bs_tops:.1f} TOP/s int8, "
-        f"{bs_bytes / (bt_ms[0] * 1e-3) / 1e9:.0f} GB/s), select {bt_ms[1] * 1e3:.1f}us, final {bt_ms[2] * 1e3:.1f}us")
+    log("[bench] batch: " + ", ".join(f"{n} {v * 1e3:.1f}us" for n, v in zip(names, bt_ms)) +
+        f"; {sc_kernel} {bs_gbs:.0f} GB/s, {bs_tops:.1f} TOP/s int8")
     log(f"[bench] single: stage1 {lb_ms[0] * 1e3:.1f}us ({s1_gbs:.0f} GB/s), merge {lb_ms[1] * 1e3:.1f}us, "
         f"final {lb_ms[2] * 1e3:.1f}us; fp32 spmv {fp_ms[0] * 1e3:.1f}us ({fp32_spmv_gbs:.0f} GB/s), "
         f"select {fp_ms[1] * 1e3:.1f}us + {fp_ms[2] * 1e3:.1f}us")
@@ -177,7 +205,7 @@ def main():
 
     # ---- HBM traffic from rocprof counters, when profiled for this workload ----
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "batch_scores_400000x500_8bit_pmc.json")
+    pmc = os.path.join(ROOT, "profiles", f"{sc_kernel}_{a.rows}x{a.cols}_{a.bits}bit_pmc.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -211,18 +239,30 @@ def main():
             "rows": a.rows, "cols": a.cols, "nnz": csr.nnz, "bits": a.bits, "split": a.split, "ck": CK,
             "k": K, "queries_per_step": B, "parallelism": "single GPU",
         },
-        "roofline": {
+        "roofline": ({
+            "bound": "hbm",
+            "kernel": sc_kernel,
+            "achieved": round(bs_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(bs_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": int(bs_bytes),
+            "avg_launch_us": round(float(t_sc) * 1e3, 2),
+            "mfma_int8_tops": round(bs_tops, 2),
+            "mfma_frac": round(bs_tops / INT8_PEAK_TOPS, 4),
+        } if dense else {
             "bound": "mfma",
-            "kernel": "batch_scores",
+            "kernel": sc_kernel,
             "achieved": round(bs_tops, 2),
             "peak": INT8_PEAK_TOPS,
             "unit": "TFLOP/s",
             "frac": round(bs_tops / INT8_PEAK_TOPS, 5),
             "traffic": traffic,
-            "algorithmic_ops_per_launch": 2 * kept * B,
-            "avg_launch_us": round(float(bt_ms[0]) * 1e3, 2),
+            "algorithmic_ops_per_launch": int(bs_ops),
+            "avg_launch_us": round(float(t_sc) * 1e3, 2),
             "note": "int8 MACs on VALU dot4 (one useful byte lane); peak is the dense int8 matrix-core figure",
-        },
+        }),
         "roofline_single_query_stage1": {
             "bound": "hbm", "kernel": "lowbit_stage1", "achieved": round(s1_gbs, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(s1_gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": s1_bytes,
@@ -233,7 +273,10 @@ def main():
         "fp32_csr_gpu_qps": round(fqps, 2),
         "lowbit_batched_vs_fp32_speedup": round(qps / fqps, 3),
         "batched_equals_single": same,
-        "batch_launch_us": [round(float(v) * 1e3, 2) for v in bt_ms],
+        "batch_launch_us": dict(zip(names, [round(float(v) * 1e3, 2) for v in bt_ms])),
+        "batch_scoring": "dense int8 tiles (MFMA)" if dense else "sliced packets (dot4)",
+        "dense_bytes": g.info.dense_bytes,
+        "batched_other_layout_qps": None if other_qps is None else round(other_qps, 2),
         "lowbit_launch_us": [round(float(v) * 1e3, 2) for v in lb_ms],
         "fp32_launch_us": [round(float(v) * 1e3, 2) for v in fp_ms],
         "fp32_spmv_gbs": round(fp32_spmv_gbs, 1),

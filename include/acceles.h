@@ -214,6 +214,9 @@ typedef struct {
     int64_t csr_bytes;     /* FP32 CSR bytes read per query                  */
     int64_t stream_bytes;  /* low-bit bytes read per stage-1 pass            */
     int64_t device_bytes;  /* total device memory held by the handle         */
+    int32_t batch_mode;    /* ACC_BATCH_* in effect                          */
+    int32_t dense_ok;      /* the dense int8 tile layout can hold the matrix */
+    int64_t dense_bytes;   /* bytes of the dense tiles (0 until built)       */
 } acc_gpu_info_t;
 
 int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info);
@@ -255,6 +258,23 @@ int acc_gpu_spmv_fp64(acc_gpu_matrix *m, const float *x_dev, double *y_dev, void
 int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq,
                               const int32_t *k_host, int ck, acc_element *out_dev,
                               int out_stride, void *stream);
+
+/* Scoring layout of the batched path.  The reference scores one query per
+ * pass (host_spmv_ucsr.cpp:1084-1180); a batch here is one pass per block of
+ * queries, either over the sliced packets (v_dot4 on the vector ALUs) or over
+ * dense int8 tiles on the matrix cores (v_mfma_i32_32x32x32_i8; 64-row slices
+ * x columns rounded up to 32, one byte per entry).  Both give the same
+ * integer scores and the same results.  ACC_BATCH_DENSE fails with a message
+ * when the layout cannot hold the matrix exactly (a row naming a column twice,
+ * |score| >= 2^25, or more than 4960 columns); ACC_BATCH_AUTO uses the dense
+ * tiles when they can hold the matrix and the batch has at least
+ * min_dense_batch queries (0 keeps the current value; default 16), building
+ * them on first use.  Setting DENSE or AUTO on an eligible matrix builds the
+ * tiles now. */
+#define ACC_BATCH_AUTO 0
+#define ACC_BATCH_SPARSE 1
+#define ACC_BATCH_DENSE 2
+int acc_gpu_set_batch_mode(acc_gpu_matrix *m, int mode, int min_dense_batch);
 
 /* Stage outputs, for parity checks.
  *   spmv_fp32:      y_dev[rows] FP32 scores (the Top-K path's order)

@@ -179,8 +179,27 @@ def test_lowbit_stream_and_host_form_agree(acc, gpu):
     np.testing.assert_array_equal(h["value"], v)
 
 
-def test_lowbit_batch_uneven_k(acc, gpu):
+BATCH_MODES = pytest.mark.parametrize("mode", [1, 2], ids=["packets", "dense"])
+
+
+def check_batch(acc, g, csr, bits, signed, X, ks, ck, split, out_stride=100):
+    """Batched Top-K through the C ABI against the oracle, every query."""
+    ptr, idx, val, q = oracle_qv(csr, bits, signed)
+    nq, cols = X.shape
+    out = empty(acc, nq * out_stride * 2, np.int32)
+    g.topk_lowbit_batch(dev(acc, X), ks, ck, out, out_stride)
+    acc.synchronize()
+    a = out.numpy().reshape(nq, out_stride, 2)
+    for qi in range(nq):
+        _, oi, ov = O.two_stage(ptr, idx, val, q, X[qi], cols, g.info.group, split, ck, int(ks[qi]))
+        np.testing.assert_array_equal(a[qi, :ks[qi], 1], oi, err_msg=f"query {qi}")
+        np.testing.assert_array_equal(a[qi, :ks[qi], 0].view(np.float32), ov, err_msg=f"query {qi}")
+
+
+@BATCH_MODES
+def test_lowbit_batch_uneven_k(acc, gpu, mode):
     csr, pk, g = make(acc, 30000, 900, (30, 90), 8, 32, seed=8)
+    g.set_batch_mode(mode)
     ptr, idx, val, q = oracle_qv(csr, 8, False)
     nq = 12
     X = acc.generate_queries(nq, 900, seed=8)
@@ -195,27 +214,66 @@ def test_lowbit_batch_uneven_k(acc, gpu):
         np.testing.assert_array_equal(a[qi, :ks[qi], 0].view(np.float32), ov)
 
 
+@BATCH_MODES
 @pytest.mark.parametrize("rows,cols,deg,bits,split,signed,nq", [
-    (20000, 500, (40, 100), 8, 32, False, 40),   # unsigned 8-bit: offset-byte dot4, 32-query blocks
+    (20000, 500, (40, 100), 8, 32, False, 40),   # unsigned 8-bit (offset bytes), 64-query blocks
     (20000, 900, (30, 90), 8, 16, True, 37),     # signed 8-bit
-    (20000, 500, (40, 100), 4, 32, False, 9),    # 4-bit, 16-query block
-    (30000, 1024, (0.5, 40.0), 8, 64, False, 33),  # power-law rows
+    (20000, 500, (40, 100), 4, 32, False, 9),    # 4-bit, one 32-query block
+    (30000, 1024, (0.5, 40.0), 8, 64, False, 33),  # power-law rows, 1024 columns
+    (12000, 300, (5, 30), 4, 8, False, 300),     # 300 queries: three 128-query blocks, a ragged last one
 ])
-def test_lowbit_batch_matches_oracle(acc, gpu, rows, cols, deg, bits, split, signed, nq):
+def test_lowbit_batch_matches_oracle(acc, gpu, mode, rows, cols, deg, bits, split, signed, nq):
     kind = 1 if isinstance(deg[0], float) else 0
     csr, pk, g = make(acc, rows, cols, deg, bits, split, signed, kind, seed=rows + nq)
-    ptr, idx, val, q = oracle_qv(csr, bits, signed)
+    g.set_batch_mode(mode)
     X = acc.generate_queries(nq, cols, seed=nq)
     ck = min(64, 512 // split) * split
     ks = np.array([KS[i % len(KS)] for i in range(nq)], np.int32)
-    out = empty(acc, nq * 100 * 2, np.int32)
-    g.topk_lowbit_batch(dev(acc, X), ks, ck, out, 100)
-    acc.synchronize()
-    a = out.numpy().reshape(nq, 100, 2)
-    for qi in range(nq):
-        _, oi, ov = O.two_stage(ptr, idx, val, q, X[qi], cols, g.info.group, split, ck, int(ks[qi]))
-        np.testing.assert_array_equal(a[qi, :ks[qi], 1], oi, err_msg=f"query {qi}")
-        np.testing.assert_array_equal(a[qi, :ks[qi], 0].view(np.float32), ov, err_msg=f"query {qi}")
+    check_batch(acc, g, csr, bits, signed, X, ks, ck, split)
+
+
+@BATCH_MODES
+def test_lowbit_batch_heavy_ties(acc, gpu, mode):
+    """2-bit values and a 0/1 query: few distinct scores, so slice lists
+    overflow at the threshold and the dense path rescans slices exactly."""
+    csr, pk, g = make(acc, 6000, 64, (20, 40), 2, 8, seed=31)
+    g.set_batch_mode(mode)
+    X = (acc.generate_queries(20, 64, seed=31) > 0).astype(np.float32)
+    ks = np.array([[1, 8, 16, 32, 50, 64][i % 6] for i in range(20)], np.int32)
+    check_batch(acc, g, csr, 2, False, X, ks, 64, 8)  # M = 8 of 12 slices per partition
+
+
+@BATCH_MODES
+def test_lowbit_batch_tiny_partitions(acc, gpu, mode):
+    """Partitions of one or two slices (at most M slices: every slice is
+    scanned), empty rows, K larger than some partitions hold."""
+    csr, pk, g = make(acc, 333, 40, (0, 6), 8, 16, seed=5)
+    g.set_batch_mode(mode)
+    X = acc.generate_queries(18, 40, seed=5)
+    ks = np.array([1, 8, 16, 32, 50, 75, 100, 100, 3] * 2, np.int32)
+    check_batch(acc, g, csr, 8, False, X, ks, 1024, 16)
+
+
+def test_dense_layout_refuses_duplicate_columns(acc, gpu):
+    """A row naming a column twice cannot be one byte per column: the dense
+    mode is refused with a message, AUTO keeps the packet path, and the
+    results still match the oracle (which sums both entries)."""
+    rng = np.random.default_rng(9)
+    rows, cols = 3000, 128
+    ptr = np.arange(0, rows * 12 + 1, 12, dtype=np.int64)
+    idx = rng.integers(0, cols, rows * 12).astype(np.int32)
+    idx[5 * 12 + 1] = idx[5 * 12]  # row 5 repeats a column
+    val = np.abs(rng.standard_normal(rows * 12)).astype(np.float32)
+    val[5 * 12:5 * 12 + 2] = 3.0  # both copies keep a non-zero quantized value
+    csr = acc.CSRMatrix.from_arrays(rows, cols, ptr, idx, val)
+    g = acc.GpuMatrix(csr, csr.pack(bits=8, split=8))
+    assert not g.info.dense_ok
+    with pytest.raises(acc.AccelesError, match="twice"):
+        g.set_batch_mode(acc.BATCH_DENSE)
+    g.set_batch_mode(acc.BATCH_AUTO)
+    X = acc.generate_queries(24, cols, seed=9)
+    ks = np.array([KS[i % len(KS)] for i in range(24)], np.int32)
+    check_batch(acc, g, csr, 8, False, X, ks, 256, 8)
 
 
 def test_fp64_exact_scores(acc, gpu):
@@ -230,7 +288,7 @@ def test_fp64_exact_scores(acc, gpu):
 
 
 # --------------------------------------------------------------------------
-# full-size configuration checks (configs[1]: 400k x 500)
+# full-size checks (400,000 x 500)
 # --------------------------------------------------------------------------
 
 def test_embedding_scale_parity_and_recall(acc, gpu):
