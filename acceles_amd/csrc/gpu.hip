@@ -416,24 +416,12 @@ int launch_dense(acc_gpu_matrix *m, const float *X, int nq, int M, int kmax, acc
     hipLaunchKernelGGL(get_batch_rescore(), dim3((unsigned)((threads + ACC_WG - 1) / ACC_WG)), dim3(ACC_WG), 0, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
     mark(m, 4, st);
-    const int total = m->split * M;
-    if (total <= 1024) {
-        dense_fn ft = get_batch_topk(pow2_at_least((total + 63) / 64));
-        hipLaunchKernelGGL(ft, dim3((nq + ACC_WAVES - 1) / ACC_WAVES), dim3(ACC_WG), 0, st, a);
-    } else {
-        BatchArgs b;
-        std::memset(&b, 0, sizeof(b));
-        b.split = m->split;
-        b.M = M;
-        b.resc = m->d_bresc;
-        b.K = m->d_bK;
-        b.out = out;
-        b.out_stride = out_stride;
-        const int R = r_for(kmax);
-        batch_fn ff = get_batch_final(R);
-        if (!ff) return fail(ACC_ERR_ARG, "k too large");
-        hipLaunchKernelGGL(ff, dim3(nq), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, b);
-    }
+    (void)kmax;
+    const int total = m->split * M;  // <= kMaxCK = 4096 = 4 lists of 64*16
+    const int R = pow2_at_least((total + 64 * ACC_WAVES - 1) / (64 * ACC_WAVES));
+    dense_fn ft = get_batch_topk(R);
+    if (!ft) return fail(ACC_ERR_ARG, "too many candidates");
+    hipLaunchKernelGGL(ft, dim3(nq), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
     mark(m, 5, st);
     return check_launch();
 }

@@ -9,57 +9,72 @@
 // scores are the same integers as the packet kernels'.  Only the four best
 // (score, row) keys of each (query, 64-row slice) leave the kernel; the
 // per-partition selection derives its threshold from the slice maxima and
-// rescans (exactly, from the packets) the rare slice whose list may have
-// dropped a qualifying row.
+// rescans exactly the rare slice whose list may have dropped a qualifying row.
 #include "kernels_common.hpp"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-// 32-bit slice keys: (score + 2^25) << 6 | (63 - row_in_slice); a larger key
-// is a higher score, then a lower row.  0 = no row.  |score| < 2^25 is
-// checked on the host before the dense layout is used.
-__device__ __forceinline__ uint32_t tk_key(int32_t score, int rho) {
-    return ((uint32_t)(score + (1 << 25)) << 6) | (uint32_t)(63 - rho);
-}
+// 32-bit slice keys: (score + 2^25) << 6 | row_in_slice; a larger key is a
+// higher score (equal scores: any fixed order will do, the lists are only a
+// filter and every candidate is re-keyed by (score, row)).  0 = no row.
+// |score| < 2^25 is checked on the host before the dense layout is used.
 __device__ __forceinline__ int32_t tk_score(uint32_t k) { return (int32_t)(k >> 6) - (1 << 25); }
-__device__ __forceinline__ int tk_rho(uint32_t k) { return 63 - (int)(k & 63u); }
+__device__ __forceinline__ int tk_rho(uint32_t k) { return (int)(k & 63u); }
 
-// keeps k0 >= k1 >= k2 >= k3 the four largest of what was offered
-__device__ __forceinline__ void top4_insert(uint32_t &k0, uint32_t &k1, uint32_t &k2, uint32_t &k3, uint32_t x) {
-    uint32_t h = max(k0, x);
-    x = min(k0, x);
-    k0 = h;
-    h = max(k1, x);
-    x = min(k1, x);
-    k1 = h;
-    h = max(k2, x);
-    x = min(k2, x);
-    k2 = h;
-    k3 = max(k3, x);
+__device__ __forceinline__ uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t d;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
 }
 
-// Exact INT32 score of this lane's row of slice s from the sliced packets
-// (the entry width is a runtime value: this runs only for rescans).
-__device__ inline int32_t slice_dot_rt(const DenseArgs &a, int64_t s, const int8_t *qx) {
-    const int lane = lane_id();
-    const uint2 sl = a.slices[s];
-    const int E = a.col_bits + a.val_bits;
-    const uint32_t cmask = (1u << a.col_bits) - 1u, vmask = (1u << a.val_bits) - 1u;
+// keeps k0 >= k1 >= k2 >= k3 the four largest of what was offered: with the
+// list ordered, the new i-th entry is the median of the old (i-1)-th, i-th
+// and x (one v_med3 each)
+__device__ __forceinline__ void top4_insert(uint32_t &k0, uint32_t &k1, uint32_t &k2, uint32_t &k3, uint32_t x) {
+    const uint32_t n3 = umed3(k2, k3, x), n2 = umed3(k1, k2, x), n1 = umed3(k0, k1, x);
+    k0 = max(k0, x);
+    k1 = n1;
+    k2 = n2;
+    k3 = n3;
+}
+
+// Exact INT32 score of this lane's row (row_in_slice = lane) of slice s for
+// one query, from the dense tiles with v_dot4_i32_i8: the row's 32 bytes of
+// column block kb are the 16-byte registers of lanes i and i+32 of tile
+// (kb, rt).  qx: the query's int8 values in LDS, plain order, cols_pad long;
+// corr: its unsigned-8-bit offset term.  Integer sums in any order are the
+// matrix cores' result.
+__device__ __forceinline__ int32_t slice_dot_dense(const DenseArgs &a, int64_t s, const int8_t *qx, int32_t corr) {
+    const int lane = lane_id(), rt = lane >> 5, i = lane & 31;
+    const size_t kstr = (size_t)a.nslices * 2048;  // bytes per column block
+    const int8_t *base = a.tiles + ((size_t)s * 2 + rt) * 1024 + i * 16;
+    const int *q4 = (const int *)qx;
     int32_t acc = 0;
-    for (int j = 0;; ++j) {
-        const int pos = j * E, w = pos >> 5, sh = pos & 31;
-        if (w >= (int)sl.y) break;
-        const uint32_t lo = a.words[((size_t)sl.x + w) * 64 + lane];
-        const uint32_t hi = (w + 1 < (int)sl.y) ? a.words[((size_t)sl.x + w + 1) * 64 + lane] : 0u;
-        const uint32_t bits = (uint32_t)((((uint64_t)hi << 32) | lo) >> sh);
-        const uint32_t v = (bits >> a.col_bits) & vmask;
-        if (v == 0u) break;  // padding: entries never hold a zero value
-        int sv = (int)v;
-        if (a.is_signed && (v >> (a.val_bits - 1))) sv = (int)(v | ~vmask);
-        acc += sv * (int)qx[bits & cmask];
+    constexpr int U = 8;
+    for (int kb0 = 0; kb0 < a.nkb; kb0 += U) {
+        int4 lo[U], hi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kb = min(kb0 + u, a.nkb - 1);
+            lo[u] = *(const int4 *)(base + kb * kstr);
+            hi[u] = *(const int4 *)(base + kb * kstr + 512);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int *qd = q4 + min(kb0 + u, a.nkb - 1) * 8;
+            if (kb0 + u >= a.nkb) continue;  // wave-uniform
+            acc = __builtin_amdgcn_sdot4(lo[u].x, qd[0], acc, false);
+            acc = __builtin_amdgcn_sdot4(lo[u].y, qd[1], acc, false);
+            acc = __builtin_amdgcn_sdot4(lo[u].z, qd[2], acc, false);
+            acc = __builtin_amdgcn_sdot4(lo[u].w, qd[3], acc, false);
+            acc = __builtin_amdgcn_sdot4(hi[u].x, qd[4], acc, false);
+            acc = __builtin_amdgcn_sdot4(hi[u].y, qd[5], acc, false);
+            acc = __builtin_amdgcn_sdot4(hi[u].z, qd[6], acc, false);
+            acc = __builtin_amdgcn_sdot4(hi[u].w, qd[7], acc, false);
+        }
     }
-    return acc;
+    return acc + corr;
 }
 
 // One wavefront per slice: decode each lane's packet stream into a 32-row
@@ -96,8 +111,8 @@ __global__ __launch_bounds__(64) void dense_build(DenseArgs a) {
             }
         }
         wave_sync();
-        uint4 *dst = (uint4 *)(a.tiles + ((size_t)s * 2 + rt) * nkb * 1024);
-        for (int i = lane; i < nkb * 64; i += 64) dst[i] = l4[i];
+        uint4 *dst = (uint4 *)a.tiles + ((size_t)s * 2 + rt) * 64;
+        for (int i = lane; i < nkb * 64; i += 64) dst[(size_t)(i >> 6) * a.nslices * 128 + (i & 63)] = l4[i];
         wave_sync();
     }
 }
@@ -155,71 +170,116 @@ template <int NT>
 __global__ __launch_bounds__(512) void dense_scores(DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int QB = 32 * NT;
-    constexpr int P = 4;
     const int nkb = a.nkb, nqb = a.nqb;
     const int qb = blockIdx.x % nqb;
     const int wx = blockIdx.x / nqb, nwx = gridDim.x / nqb;
     v4i *qt = (v4i *)smem;
     int32_t *corr = (int32_t *)(smem + (size_t)nkb * NT * 1024);
     {
+        // all loads of a round are issued before any LDS write, so the table
+        // arrives in one memory latency per 64 KB instead of one per 8 KB
         const v4i *src = (const v4i *)(a.qtab + (size_t)qb * nkb * NT * 1024);
-        for (int i = threadIdx.x; i < nkb * NT * 64; i += 512) qt[i] = src[i];
+        const int n = nkb * NT * 64;
+        constexpr int CU = 8;
+        for (int b0 = 0; b0 < n; b0 += 512 * CU) {
+            v4i tmp[CU];
+#pragma unroll
+            for (int u = 0; u < CU; ++u) {
+                const int i = b0 + u * 512 + (int)threadIdx.x;
+                tmp[u] = i < n ? src[i] : (v4i){0};
+            }
+#pragma unroll
+            for (int u = 0; u < CU; ++u) {
+                const int i = b0 + u * 512 + (int)threadIdx.x;
+                if (i < n) qt[i] = tmp[u];
+            }
+        }
         for (int i = threadIdx.x; i < QB; i += 512) corr[i] = a.corr[qb * QB + i];
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = lane_id(), hi = lane >> 5;
-    for (int64_t s = (int64_t)wx * 8 + wave; s < a.nslices; s += (int64_t)nwx * 8) {
-        const v4i *A0 = (const v4i *)(a.tiles + (size_t)s * 2 * nkb * 1024) + lane;
-        const v4i *A1 = A0 + (size_t)nkb * 64;
+    const int64_t stride = (int64_t)nwx * 8;
+    int64_t s = (int64_t)wx * 8 + wave;
+    // Matrix blocks stream through a ring of RING register slots: the block of
+    // step k sits in slot k % RING and is loaded DIST steps ahead, into the
+    // slot whose MFMAs were issued RING - DIST steps before.  Loads are
+    // unconditional (indices clamped into the matrix; the few past the end are
+    // redundant re-reads) so each lands directly in its slot -- a conditional
+    // load would be copied into place after waiting for every outstanding load.
+    // The next slice's first DIST blocks are requested before the epilogue.
+    constexpr int RING = 8, DIST = 6;
+    v4i pa[RING], pb[RING];
+#pragma unroll
+    for (int j = 0; j < RING; ++j) {
+        pa[j] = (v4i){0};
+        pb[j] = (v4i){0};
+    }
+    // column block kb of all slices is contiguous: at a given step the
+    // wavefronts of the whole chip read neighbouring kilobytes
+    const size_t kstr = (size_t)a.nslices * 128;  // v4i per column block
+    const v4i *tiles = (const v4i *)a.tiles + lane;
+    if (s < a.nslices) {
+        const v4i *B0 = tiles + (size_t)s * 128;
+#pragma unroll
+        for (int j = 0; j < DIST; ++j) {
+            const size_t kj = (size_t)min(j, nkb - 1) * kstr;
+            pa[j] = __builtin_nontemporal_load(B0 + kj);
+            pb[j] = __builtin_nontemporal_load(B0 + kj + 64);
+        }
+    }
+    for (; s < a.nslices; s += stride) {
+        const v4i *A0 = tiles + (size_t)s * 128;
+        const v4i *A1 = A0 + 64;
         v16i acc[2][NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             acc[0][t] = (v16i){0};
             acc[1][t] = (v16i){0};
         }
-        v4i pa[P], pb[P];
+        for (int k0 = 0; k0 < nkb; k0 += RING) {
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            pa[p] = (v4i){0};
-            pb[p] = (v4i){0};
-            if (p < nkb) {
-                pa[p] = __builtin_nontemporal_load(A0 + p * 64);
-                pb[p] = __builtin_nontemporal_load(A1 + p * 64);
-            }
-        }
-        for (int k0 = 0; k0 < nkb; k0 += P) {
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                const int k = k0 + p;
+            for (int j = 0; j < RING; ++j) {
+                const int k = k0 + j;
+                const size_t kn = (size_t)min(k + DIST, nkb - 1) * kstr;
+                pa[(j + DIST) % RING] = __builtin_nontemporal_load(A0 + kn);
+                pb[(j + DIST) % RING] = __builtin_nontemporal_load(A1 + kn);
                 if (k < nkb) {
-                    const v4i x0 = pa[p], x1 = pb[p];
-                    if (k + P < nkb) {
-                        pa[p] = __builtin_nontemporal_load(A0 + (k + P) * 64);
-                        pb[p] = __builtin_nontemporal_load(A1 + (k + P) * 64);
-                    }
 #pragma unroll
                     for (int t = 0; t < NT; ++t) {
                         const v4i b = qt[(k * NT + t) * 64 + lane];
-                        acc[0][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(x0, b, acc[0][t], 0, 0, 0);
-                        acc[1][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(x1, b, acc[1][t], 0, 0, 0);
+                        acc[0][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(pa[j], b, acc[0][t], 0, 0, 0);
+                        acc[1][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(pb[j], b, acc[1][t], 0, 0, 0);
                     }
                 }
             }
         }
-        // D[i][j] of a 32x32 tile: lane j + 32*((i/4)%2), register (i%4) + 4*(i/8)
-        const int nv = a.nvalid[s];
+        {
+            const int64_t sn = min(s + stride, a.nslices - 1);  // past the end: a redundant in-bounds load
+            const v4i *B0 = tiles + (size_t)sn * 128;
+#pragma unroll
+            for (int j = 0; j < DIST; ++j) {
+                const size_t kj = (size_t)min(j, nkb - 1) * kstr;
+                pa[j] = __builtin_nontemporal_load(B0 + kj);
+                pb[j] = __builtin_nontemporal_load(B0 + kj + 64);
+            }
+        }
+        // D[i][j] of a 32x32 tile: lane j + 32*((i/4)%2), register (i%4) + 4*(i/8);
+        // row_in_slice = rt*32 + i = E(rt, r) + 4*hi.  Padding rows (the tail of a
+        // partition's last slice) hold no entries and score exactly 0 in every
+        // value mode; the selection discards their keys.
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int ql = t * 32 + (lane & 31);
-            const int32_t cr = corr[ql];
+            // key = ((score + corr + 2^25) << 6) + row_in_slice, in 32-bit wrapping
+            // arithmetic (the true value is in [64, 2^32))
+            const uint32_t D = ((uint32_t)(corr[ql] + (1 << 25)) << 6) + (uint32_t)(hi * 4);
             uint32_t k0 = 0u, k1 = 0u, k2 = 0u, k3 = 0u;
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const int rho = rt * 32 + (r >> 2) * 8 + hi * 4 + (r & 3);
-                    const uint32_t key = rho < nv ? tk_key(acc[rt][t][r] + cr, rho) : 0u;
-                    top4_insert(k0, k1, k2, k3, key);
+                    const uint32_t E = (uint32_t)(rt * 32 + (r >> 2) * 8 + (r & 3));
+                    top4_insert(k0, k1, k2, k3, ((uint32_t)acc[rt][t][r] << 6) + D + E);
                 }
             const uint32_t o0 = (uint32_t)__shfl_xor((int)k0, 32), o1 = (uint32_t)__shfl_xor((int)k1, 32);
             const uint32_t o2 = (uint32_t)__shfl_xor((int)k2, 32), o3 = (uint32_t)__shfl_xor((int)k3, 32);
@@ -233,17 +293,103 @@ __global__ __launch_bounds__(512) void dense_scores(DenseArgs a) {
     }
 }
 
+// Bitonic sort, descending, of 64*R 32-bit keys in registers (element
+// i = r*64 + lane in v[r] of that lane).
+template <int R>
+__device__ __forceinline__ void reg_sort_desc_u32(uint32_t (&v)[R]) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int k = 2; k <= 64 * R; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int rp = r ^ (j >> 6);
+                    if (rp > r) {
+                        const bool desc = ((r * 64 + lane) & k) == 0;
+                        const uint32_t x = v[r], y = v[rp];
+                        const bool sw = desc ? (x < y) : (x > y);
+                        v[r] = sw ? y : x;
+                        v[rp] = sw ? x : y;
+                    }
+                }
+            } else {
+                const bool lower = (lane & j) == 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t o = (uint32_t)__shfl_xor((int)v[r], j);
+                    const bool desc = ((r * 64 + lane) & k) == 0;
+                    v[r] = (lower == desc) ? max(v[r], o) : min(v[r], o);
+                }
+            }
+        }
+    }
+}
+
+// Offers the listed keys of 64*U slices (k4[u] = the list of slice
+// s0 + b + u*64 + lane) scoring >= ts to the collector; a slice whose list
+// may have dropped such a row is rescanned exactly (from the dense tiles).
+// nv[u]: valid rows of those slices (0 past the partition); keys of padding
+// rows (row_in_slice >= nv) are discarded, and since they may have displaced
+// real rows from a list, every list whose fourth key scores >= ts is rescanned.
+template <int U>
+__device__ __forceinline__ void offer_lists(const DenseArgs &a, WaveTopM<2> &cb, const uint4 (&k4)[U], const int (&nv)[U],
+                                   int64_t s0, int b, int32_t ts, int q, int8_t *qx, bool &have_qx) {
+    const int lane = lane_id();
+    bool rescan[U];
+    int32_t row[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t s = s0 + b + u * 64 + lane;
+        rescan[u] = nv[u] > 0 && tk_score(k4[u].w) >= ts;
+        const uint32_t ke[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const bool pass = !rescan[u] && nv[u] > 0 && tk_rho(ke[e]) < nv[u] && tk_score(ke[e]) >= ts;
+            row[u][e] = pass ? a.perm[s * 64 + tk_rho(ke[e])] : -1;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t ke[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) cb.push(row[u][e] >= 0, dkey_i32(tk_score(ke[e]), row[u][e]));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        uint64_t mask = __ballot(rescan[u]);
+        while (mask) {
+            const int l = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            if (!have_qx) {
+                const int8_t *src = a.qplain + (size_t)q * a.cols_pad;
+                for (int c = lane; c < a.cols_pad; c += 64) qx[c] = src[c];
+                wave_sync();
+                have_qx = true;
+            }
+            const int64_t s = s0 + b + u * 64 + l;
+            const int32_t sc = slice_dot_dense(a, s, qx, a.corr[q]);
+            const int32_t r = a.perm[s * 64 + lane];
+            cb.push(r >= 0 && sc >= ts, dkey_i32(sc, r));
+        }
+    }
+}
+
 // One wavefront per (partition p = blockIdx.x, query q = blockIdx.y*4 + wave):
 //  1. ts = the M-th best slice maximum of the partition (INT32_MIN when the
 //     partition has at most M slices).  M distinct rows score >= ts, so the
 //     partition's M-th best score is >= ts and no row below ts can be in its
 //     Top-M.
-//  2. Every listed key scoring >= ts is a candidate.  A list with four keys
-//     whose fourth still scores >= ts (and a slice of more than four rows)
-//     may have dropped a qualifying row: that slice is rescanned exactly from
-//     the packets instead.
+//  2. Every listed key of a real row scoring >= ts is a candidate.  A list
+//     whose fourth key still scores >= ts may have dropped a qualifying row:
+//     that slice is rescanned exactly (all 64 rows, one dot4 pass) instead.  Slice maxima
+//     of partial slices (padding rows score 0) are left out of step 1; the
+//     maxima of the full slices still give M distinct real rows >= ts.
 //  3. Top-M of the candidates by (score, row) (quantify_util.py:173-196 with
 //     the lower-row tie rule), written as keys for rescoring.
+// Partitions of at most 256 slices keep their lists in registers and find ts
+// with one register sort; larger ones stream the lists twice.
 __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x >> 6, lane = lane_id();
@@ -255,10 +401,35 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
     const int nsl = (int)(a.part_slice[p + 1] - s0);
     const uint4 *L = (const uint4 *)a.tk + (size_t)q * a.nslices + s0;
     const int M = a.M;
-    constexpr int U = 4;
+    WaveTopM<2> cb;
+    cb.init(bufB, M);
+    bool have_qx = false;
 
-    int32_t ts = INT32_MIN;
-    if (nsl > M) {
+    if (nsl <= 256) {
+        uint4 k4[4];
+        int nv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = u * 64 + lane;
+            k4[u] = i < nsl ? L[i] : make_uint4(0u, 0u, 0u, 0u);
+            nv[u] = i < nsl ? (int)a.nvalid[s0 + i] : 0;
+        }
+        int32_t ts = INT32_MIN;
+        if (nsl > M) {
+            uint32_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = nv[u] == 64 ? k4[u].x : 0u;
+            reg_sort_desc_u32<4>(v);
+            uint32_t t = 0u;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (r == (M - 1) >> 6) t = v[r];
+            t = (uint32_t)__shfl((int)t, (M - 1) & 63);
+            if (t != 0u) ts = tk_score(t);
+        }
+        offer_lists<4>(a, cb, k4, nv, s0, 0, ts, q, qx, have_qx);
+    } else {
+        constexpr int U = 4;
         WaveTopM<2> ca;
         ca.init(bufA, M);
         for (int b = 0; b < nsl; b += 64 * U) {
@@ -266,61 +437,23 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = b + u * 64 + lane;
-                k[u] = i < nsl ? L[i].x : 0u;
+                k[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) ca.push(k[u] != 0u, ((uint64_t)k[u] << 32) | (uint32_t)(b + u * 64 + lane));
         }
         ca.compact();
-        if (ca.n >= M) ts = tk_score((uint32_t)(ca.buf[M - 1] >> 32));
-    }
-
-    WaveTopM<2> cb;
-    cb.init(bufB, M);
-    bool have_qx = false;
-    for (int b = 0; b < nsl; b += 64 * U) {
-        uint4 k4[U];
+        const int32_t ts = ca.n >= M ? tk_score((uint32_t)(ca.buf[M - 1] >> 32)) : INT32_MIN;
+        for (int b = 0; b < nsl; b += 64 * U) {
+            uint4 k4[U];
+            int nv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int i = b + u * 64 + lane;
-            k4[u] = i < nsl ? L[i] : make_uint4(0u, 0u, 0u, 0u);
-        }
-        bool rescan[U];
-        int32_t row[U][4];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t s = s0 + b + u * 64 + lane;
-            rescan[u] = k4[u].w != 0u && tk_score(k4[u].w) >= ts && a.nvalid[s] > 4;
-            const uint32_t ke[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const bool pass = !rescan[u] && ke[e] != 0u && tk_score(ke[e]) >= ts;
-                row[u][e] = pass ? a.perm[s * 64 + tk_rho(ke[e])] : -1;
+            for (int u = 0; u < U; ++u) {
+                const int i = b + u * 64 + lane;
+                k4[u] = i < nsl ? L[i] : make_uint4(0u, 0u, 0u, 0u);
+                nv[u] = i < nsl ? (int)a.nvalid[s0 + i] : 0;
             }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t ke[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
-#pragma unroll
-            for (int e = 0; e < 4; ++e) cb.push(row[u][e] >= 0, dkey_i32(tk_score(ke[e]), row[u][e]));
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            uint64_t mask = __ballot(rescan[u]);
-            while (mask) {
-                const int l = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                if (!have_qx) {
-                    const int8_t *src = a.qplain + (size_t)q * a.cols_pad;
-                    for (int c = lane; c < a.cols_pad; c += 64) qx[c] = src[c];
-                    wave_sync();
-                    have_qx = true;
-                }
-                const int64_t s = s0 + b + u * 64 + l;
-                const int32_t sc = slice_dot_rt(a, s, qx);
-                const int32_t r = a.perm[s * 64 + lane];
-                cb.push(r >= 0 && sc >= ts, dkey_i32(sc, r));
-            }
+            offer_lists<U>(a, cb, k4, nv, s0, b, ts, q, qx, have_qx);
         }
     }
     cb.compact();
@@ -351,33 +484,68 @@ __global__ __launch_bounds__(ACC_WG) void batch_rescore(DenseArgs a) {
     if (in && (threadIdx.x & (G - 1)) == 0) a.resc[cid] = valid ? dkey_f32(sc, r) : 0ull;
 }
 
-// One wavefront per query: register bitonic sort of its split*M <= 64*R
-// rescored keys, the first K[q] written as ranked results.
+// One workgroup per query: each wavefront sorts a quarter of the query's
+// split*M rescored keys in registers (64*R keys), the four sorted lists meet in
+// LDS, and every key finds its rank in the union by binary search in the other
+// three lists (keys are distinct: one per row).  Ranks below K[q] are written.
 template <int R>
 __global__ __launch_bounds__(ACC_WG) void batch_topk(DenseArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_nz[ACC_WAVES];
+    constexpr int N = 64 * R;  // keys per wavefront list
+    uint64_t *lists = (uint64_t *)smem;
     const int wave = threadIdx.x >> 6, lane = lane_id();
-    const int q = blockIdx.x * ACC_WAVES + wave;
-    if (q >= a.nq) return;
+    const int q = blockIdx.x;
     const int total = a.split * a.M;
     const uint64_t *keys = a.resc + (size_t)q * total;
     uint64_t v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int i = r * 64 + lane;
+        const int i = wave * N + r * 64 + lane;
         v[r] = i < total ? keys[i] : 0ull;
     }
     reg_sort_desc<R>(v);
+    int nz = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        lists[wave * N + r * 64 + lane] = v[r];
+        nz += v[r] != 0ull;
+    }
+    for (int o = 32; o > 0; o >>= 1) nz += __shfl_xor(nz, o);
+    if (lane == 0) s_nz[wave] = nz;
+    __syncthreads();
     const int K = a.K[q];
     acc_element *out = a.out + (size_t)q * a.out_stride;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int i = r * 64 + lane;
-        if (i < K) {
-            acc_element e;
-            e.value = v[r] ? dkey_score_f32(v[r]) : -__builtin_inff();
-            e.index = v[r] ? dkey_row(v[r]) : -1;
-            out[i] = e;
+        const uint64_t key = v[r];
+        if (key == 0ull) continue;
+        int rank = r * 64 + lane;  // its place in its own (sorted) list
+#pragma unroll
+        for (int w = 0; w < ACC_WAVES; ++w) {
+            if (w == wave) continue;
+            const uint64_t *L = lists + w * N;
+            int lo = 0, hi = N;  // number of keys of L above key
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (L[mid] > key) lo = mid + 1;
+                else hi = mid;
+            }
+            rank += lo;
         }
+        if (rank < K) {
+            acc_element e;
+            e.value = dkey_score_f32(key);
+            e.index = dkey_row(key);
+            out[rank] = e;
+        }
+    }
+    const int n = s_nz[0] + s_nz[1] + s_nz[2] + s_nz[3];
+    for (int i = n + (int)threadIdx.x; i < K; i += ACC_WG) {
+        acc_element e;
+        e.value = -__builtin_inff();
+        e.index = -1;
+        out[i] = e;
     }
 }
 

@@ -142,13 +142,29 @@ __device__ inline void publish_and_append(WaveTopM<R> &col, uint64_t *thr, uint3
 
 // Dot product of CSR row r with x by a group of G lanes (runtime G): lane g
 // sums entries g, g+G, ... with fmaf, then an xor butterfly of plain adds.
+// Entries are fetched four at a time (then their x values), so a row costs
+// one memory round trip per 4*G entries; the fmaf order is unchanged.
 __device__ __forceinline__ float group_dot_rt(const uint32_t *ptr, const int32_t *idx, const float *val, const float *x,
                                               int r, bool valid, int G) {
     const int g = lane_id() & (G - 1);
     float p = 0.f;
     if (valid) {
         const uint32_t e = ptr[r + 1];
-        for (uint32_t j = ptr[r] + g; j < e; j += G) p = __builtin_fmaf(val[j], x[idx[j]], p);
+        for (uint32_t j = ptr[r] + g; j < e; j += 4 * G) {
+            float v[4], xv[4];
+            int32_t c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t jj = j + u * G;
+                v[u] = jj < e ? val[jj] : 0.f;
+                c[u] = jj < e ? idx[jj] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (j + u * G < e) p = __builtin_fmaf(v[u], xv[u], p);
+        }
     }
     for (int s = G >> 1; s >= 1; s >>= 1) p = p + __shfl_xor(p, s);
     return p;
@@ -274,7 +290,8 @@ struct QTable {
 //   batch_topk     per query: its Top-K[q] of the rescored keys.
 //
 // Tile layout: tile (rt, kb) of slice s (rows s*64 + rt*32 .. +31, columns
-// kb*32 .. +31) is 1 KB at ((s*2 + rt)*nkb + kb)*1024; lane l's 16 bytes hold
+// kb*32 .. +31) is 1 KB at ((kb*nslices + s)*2 + rt)*1024 (column-block
+// major, so concurrent wavefronts stream neighbouring memory); lane l's 16 bytes hold
 // row l%32, columns kb*32 + (l/32)*16 + 0..15 -- the A-operand register of
 // the MFMA.  The query table uses the same lane mapping for the B operand
 // (query l%32 of a 32-query tile), so every k is paired identically in both.
