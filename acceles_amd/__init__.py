@@ -30,6 +30,7 @@ ERRORS = {
 DEG_UNIFORM, DEG_GAMMA = 0, 1
 VAL_ABSNORMAL, VAL_NORMAL, VAL_UNIFORM = 0, 1, 2
 BATCH_AUTO, BATCH_SPARSE, BATCH_DENSE = 0, 1, 2
+PLAN_LOWBIT, PLAN_LOWBIT_BATCH, PLAN_FP32 = 1, 2, 3
 
 
 class AccelesError(RuntimeError):
@@ -120,6 +121,9 @@ SIGNATURES = [
     ("acc_gpu_matrix_destroy", None, [_P]),
     ("acc_gpu_set_profiling", C.c_int, [_P, C.c_int]),
     ("acc_gpu_set_batch_mode", C.c_int, [_P, C.c_int, C.c_int]),
+    ("acc_gpu_plan_create", C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, C.POINTER(_P)]),
+    ("acc_gpu_plan_launch", C.c_int, [_P, _P]),
+    ("acc_gpu_plan_destroy", None, [_P]),
     ("acc_gpu_stage_times", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
     ("acc_gpu_topk_fp32", C.c_int, [_P, _P, C.c_int, _P, _P]),
     ("acc_gpu_topk_lowbit", C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P]),
@@ -472,6 +476,34 @@ def _stream(stream):
     if isinstance(stream, Stream):
         return stream.handle
     return _P(stream if isinstance(stream, int) else stream.cuda_stream)
+
+
+class Plan:
+    """A query launch sequence captured into a HIP graph (acc_gpu_plan_create).
+
+    Reads the device buffer x and writes out on every launch; refill x between
+    launches.  The GpuMatrix must outlive the plan."""
+
+    def __init__(self, matrix, kind, x, ks, ck, out, out_stride):
+        ks = _i32(np.atleast_1d(ks))
+        h = _P()
+        _check(lib().acc_gpu_plan_create(matrix._h, int(kind), _ptr(x), len(ks), ks.ctypes.data, int(ck), _ptr(out),
+                                         int(out_stride), C.byref(h)))
+        self._h, self._m = h, matrix
+
+    def launch(self, stream=None):
+        _check(lib().acc_gpu_plan_launch(self._h, _stream(stream)))
+
+    def close(self):
+        if self._h:
+            lib().acc_gpu_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class GpuMatrix:

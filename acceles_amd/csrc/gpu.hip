@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -24,7 +25,33 @@ using acc::fail;
 static constexpr int kMaxM = 64;     // rows kept per partition
 static constexpr int kMaxK = 960;    // Top-K limit of the FP32 path (collector cap 1024)
 static constexpr int kMaxCK = 4096;  // candidates of the low-bit path
+static constexpr size_t kWordPad = 32 * 64;  // dwords past the packets (>= widest group, 22 x 64)
 static constexpr size_t kDenseLds = 160 * 1024 - 1024;  // LDS of a dense_scores workgroup
+static constexpr int64_t kMallBytes = 192ll << 20;      // dense tiles that stay cache resident (256 MB MALL)
+
+// Per-batch device workspace of the batched path.  The handle owns one;
+// every plan (captured graph) owns its own, so replaying a plan never touches
+// buffers that a later call on the handle may reallocate.
+struct BatchWs {
+    std::vector<void *> *allocs = nullptr;  // owner's allocation list
+    int64_t *bytes = nullptr;               // owner's byte count
+    int32_t *K = nullptr;
+    size_t K_n = 0;
+    int32_t *S = nullptr;        // packet layout: nq x slots INT32 scores
+    size_t S_n = 0;
+    uint64_t *resc = nullptr;    // nq x split*M rescored keys
+    size_t resc_n = 0;
+    int8_t *qtab = nullptr;      // dense layout: B-operand table
+    size_t qtab_n = 0;
+    int8_t *qplain = nullptr;
+    size_t qplain_n = 0;
+    int32_t *corr = nullptr;
+    size_t corr_n = 0;
+    uint32_t *tk = nullptr;      // four-key slice lists
+    size_t tk_n = 0;
+    uint64_t *ckeys = nullptr;   // per (query, partition) candidates
+    size_t ckeys_n = 0;
+};
 
 struct acc_gpu_matrix {
     int device = 0;
@@ -59,13 +86,7 @@ struct acc_gpu_matrix {
     uint64_t *d_resc = nullptr;      // split * kMaxM
     int32_t *d_crow = nullptr, *d_cscore = nullptr;  // split * kMaxM (candidate outputs)
     int64_t *d_part_slice = nullptr;                 // split + 1
-    // batched-path workspace (grown on demand, freed with the handle)
-    int32_t *d_bS = nullptr;
-    size_t bS_elems = 0;
-    uint64_t *d_bresc = nullptr;
-    size_t bresc_elems = 0;
-    int32_t *d_bK = nullptr;
-    int bK_elems = 0;
+    BatchWs ws;                      // batched-path workspace (grown on demand)
     // dense int8 tiles of the batched path (matrix cores), built on demand
     uint8_t *d_nvalid = nullptr;     // valid rows per slice
     bool dense_ok = false;           // the layout can hold this matrix exactly
@@ -76,16 +97,6 @@ struct acc_gpu_matrix {
     int cols_pad = 0, nkb = 0, u8off = 0;
     int8_t *d_tiles = nullptr;
     int64_t dense_bytes = 0;
-    int8_t *d_qtab = nullptr;
-    size_t qtab_elems = 0;
-    int8_t *d_qplain = nullptr;
-    size_t qplain_elems = 0;
-    int32_t *d_corr = nullptr;
-    size_t corr_elems = 0;
-    uint32_t *d_tk = nullptr;
-    size_t tk_elems = 0;
-    uint64_t *d_ckeys = nullptr;
-    size_t ckeys_elems = 0;
     // profiling events around the launches of the last query
     bool profiling = false;
     hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -121,20 +132,30 @@ void free_all(acc_gpu_matrix *m) {
         }
 }
 
-// Grows a handle-owned workspace buffer to at least `need` elements.
+// Grows a workspace buffer to at least `need` elements (the allocation is
+// recorded in the owner's list).
 template <typename T>
-int grow(acc_gpu_matrix *m, T **p, size_t *cap, size_t need) {
+int grow_in(std::vector<void *> &allocs, int64_t &bytes, T **p, size_t *cap, size_t need) {
     if (*p && *cap >= need) return ACC_OK;
     if (*p) {
         hipFree(*p);
-        m->allocs.erase(std::find(m->allocs.begin(), m->allocs.end(), (void *)*p));
-        m->device_bytes -= (int64_t)(*cap * sizeof(T));
+        allocs.erase(std::find(allocs.begin(), allocs.end(), (void *)*p));
+        bytes -= (int64_t)(*cap * sizeof(T));
         *p = nullptr;
         *cap = 0;
     }
-    int rc = dalloc(m, p, need);
-    if (rc == ACC_OK) *cap = need;
-    return rc;
+    const size_t n = std::max<size_t>(need, 1);
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(ACC_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(*p);
+    bytes += (int64_t)(n * sizeof(T));
+    *cap = n;
+    return ACC_OK;
+}
+
+template <typename T>
+int ws_grow(BatchWs &ws, T **p, size_t *cap, size_t need) {
+    return grow_in(*ws.allocs, *ws.bytes, p, cap, need);
 }
 
 // Records profiling event i on the query's stream (when profiling is on).
@@ -176,7 +197,7 @@ std::vector<int4> make_items(const acc_packed *pk, int cus, std::vector<int32_t>
         int64_t b = s0, acc = 0;
         for (int64_t s = s0; s < s1; ++s) {
             acc += pk->slice_ndw[(size_t)s] + kSliceCost;
-            if (acc >= per || s + 1 == s1) {
+            if (acc >= per || s + 1 == s1 || s + 1 - b >= LB_MAX_ITEM) {
                 items.push_back(make_int4(p, (int)b, (int)(s + 1), 0));
                 part_wgs[(size_t)p]++;
                 b = s + 1;
@@ -246,11 +267,11 @@ int launch_lowbit(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *
     return check_launch();
 }
 
-// Batched low-bit queries: batch_scores over query blocks, batch_select per
-// (partition, query), batch_final per query (skipped when out is null).
-// Workspace (d_bS, d_bresc, d_bK) must already hold nq queries.
-int launch_batch(acc_gpu_matrix *m, const float *X, int nq, int M, int kmax, acc_element *out, int out_stride,
-                 int32_t *crow, int32_t *cscore, hipStream_t st) {
+// Batched low-bit queries on the packets: batch_scores over query blocks,
+// batch_select per (partition, query), batch_final per query (skipped when out
+// is null).  The workspace must already hold nq queries (S, resc, K).
+int launch_batch(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, int kmax, acc_element *out,
+                 int out_stride, int32_t *crow, int32_t *cscore, hipStream_t st) {
     BatchArgs a;
     a.words = m->d_words;
     a.slices = m->d_slices;
@@ -263,15 +284,15 @@ int launch_batch(acc_gpu_matrix *m, const float *X, int nq, int M, int kmax, acc
     a.col_bits = m->col_bits;
     a.val_bits = m->bits;
     a.nslots = m->nslices * 64;
-    a.S = m->d_bS;
+    a.S = ws.S;
     a.split = m->split;
     a.M = M;
     a.ptr = m->d_ptr;
     a.idx = m->d_idx;
     a.val = m->d_val;
     a.G = m->group;
-    a.resc = out ? m->d_bresc : nullptr;
-    a.K = m->d_bK;
+    a.resc = out ? ws.resc : nullptr;
+    a.K = ws.K;
     a.out = out;
     a.out_stride = out_stride;
     a.cand_rows_out = crow;
@@ -301,10 +322,18 @@ int launch_batch(acc_gpu_matrix *m, const float *X, int nq, int M, int kmax, acc
 }
 
 // Raises a kernel's dynamic LDS limit when it needs more than 64 KB.
+// Only raised once per kernel and size (also keeps the call out of graph
+// capture, where the first, eager run of a plan has already made it).
 int lds_limit(const void *fn, size_t bytes) {
     if (bytes <= 65536) return ACC_OK;
+    static std::mutex mu;
+    static std::vector<std::pair<const void *, size_t>> done;
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto &d : done)
+        if (d.first == fn && d.second >= bytes) return ACC_OK;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return fail(ACC_ERR_DEVICE, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+    done.push_back({fn, bytes});
     return ACC_OK;
 }
 
@@ -360,8 +389,8 @@ bool use_dense(const acc_gpu_matrix *m, int nq) {
 
 // Batched low-bit queries on the dense tiles: dense_prep, dense_scores,
 // dense_select, then (out != null) batch_rescore and batch_topk.
-int launch_dense(acc_gpu_matrix *m, const float *X, int nq, int M, int kmax, acc_element *out, int out_stride,
-                 int32_t *crow, int32_t *cscore, hipStream_t st) {
+int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, int kmax, acc_element *out,
+                 int out_stride, int32_t *crow, int32_t *cscore, hipStream_t st) {
     int rc = build_dense(m);
     if (rc != ACC_OK) return rc;
     int NT = nq <= 32 ? 1 : (nq <= 64 ? 2 : 4);
@@ -369,26 +398,26 @@ int launch_dense(acc_gpu_matrix *m, const float *X, int nq, int M, int kmax, acc
     const int QB = 32 * NT;
     const int nqb = (nq + QB - 1) / QB;
     const size_t cand = (size_t)nq * m->split * M;
-    rc = grow(m, &m->d_qtab, &m->qtab_elems, (size_t)nqb * m->nkb * NT * 1024);
-    if (rc == ACC_OK) rc = grow(m, &m->d_qplain, &m->qplain_elems, (size_t)nq * m->cols_pad);
-    if (rc == ACC_OK) rc = grow(m, &m->d_corr, &m->corr_elems, (size_t)nqb * QB);
-    if (rc == ACC_OK) rc = grow(m, &m->d_tk, &m->tk_elems, (size_t)nq * m->nslices * 4);
-    if (rc == ACC_OK) rc = grow(m, &m->d_ckeys, &m->ckeys_elems, cand);
-    if (rc == ACC_OK) rc = grow(m, &m->d_bresc, &m->bresc_elems, cand);
+    rc = ws_grow(ws, &ws.qtab, &ws.qtab_n, (size_t)nqb * m->nkb * NT * 1024);
+    if (rc == ACC_OK) rc = ws_grow(ws, &ws.qplain, &ws.qplain_n, (size_t)nq * m->cols_pad);
+    if (rc == ACC_OK) rc = ws_grow(ws, &ws.corr, &ws.corr_n, (size_t)nqb * QB);
+    if (rc == ACC_OK) rc = ws_grow(ws, &ws.tk, &ws.tk_n, (size_t)nq * m->nslices * 4);
+    if (rc == ACC_OK) rc = ws_grow(ws, &ws.ckeys, &ws.ckeys_n, cand);
+    if (rc == ACC_OK) rc = ws_grow(ws, &ws.resc, &ws.resc_n, cand);
     if (rc != ACC_OK) return rc;
     DenseArgs a = dense_args(m);
     a.X = X;
     a.nq = nq;
     a.NT = NT;
     a.nqb = nqb;
-    a.qtab = m->d_qtab;
-    a.qplain = m->d_qplain;
-    a.corr = m->d_corr;
-    a.tk = m->d_tk;
+    a.qtab = ws.qtab;
+    a.qplain = ws.qplain;
+    a.corr = ws.corr;
+    a.tk = ws.tk;
     a.M = M;
-    a.ckeys = m->d_ckeys;
-    a.resc = m->d_bresc;
-    a.K = m->d_bK;
+    a.ckeys = ws.ckeys;
+    a.resc = ws.resc;
+    a.K = ws.K;
     a.out = out;
     a.out_stride = out_stride;
     a.cand_rows_out = crow;
@@ -398,7 +427,7 @@ int launch_dense(acc_gpu_matrix *m, const float *X, int nq, int M, int kmax, acc
     hipLaunchKernelGGL(get_dense_prep(), dim3(nqb * QB), dim3(ACC_WG), 0, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
     mark(m, 1, st);
-    dense_fn fs = get_dense_scores(NT);
+    dense_fn fs = get_dense_scores(NT, m->dense_bytes > kMallBytes);
     const size_t lds_s = (size_t)m->nkb * NT * 1024 + (size_t)QB * 4;
     if ((rc = lds_limit((const void *)fs, lds_s)) != ACC_OK) return rc;
     const int64_t groups = (m->nslices + 7) / 8;
@@ -563,6 +592,8 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
     if (cols > 8192) return fail(ACC_ERR_ARG, "cols must be <= 8192");
     acc_gpu_matrix *m = new acc_gpu_matrix();
     m->device = device;
+    m->ws.allocs = &m->allocs;
+    m->ws.bytes = &m->device_bytes;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) m->cus = prop.multiProcessorCount;
     m->rows = csr ? csr->rows : pk->rows;
@@ -631,7 +662,9 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         coff.assign((size_t)m->split + 1, 0);
         for (int p = 0; p < m->split; ++p) coff[(size_t)p + 1] = coff[(size_t)p] + (int64_t)part_wgs[(size_t)p] * kMaxM;
         m->stream_bytes = (int64_t)pk->words.size() * 4 + m->nslices * 8;
-        TRY(dalloc(m, &m->d_words, pk->words.size()));
+        // E*64 dwords of zero padding: the packet kernels load whole groups
+        // unconditionally and mask what lies past a group's length
+        TRY(dalloc(m, &m->d_words, pk->words.size() + kWordPad));
         TRY(dalloc(m, &m->d_slices, sl.size()));
         TRY(dalloc(m, &m->d_perm, pk->perm.size()));
         TRY(dalloc(m, &m->d_items, items.size()));
@@ -645,6 +678,7 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         TRY(dalloc(m, &m->d_part_slice, (size_t)m->split + 1));
         COPY(m->d_part_slice, pk->part_slice_begin.data(), ((size_t)m->split + 1) * 8);
         COPY(m->d_words, pk->words.data(), pk->words.size() * 4);
+        ZERO(m->d_words + pk->words.size(), (size_t)kWordPad * 4);
         COPY(m->d_slices, sl.data(), sl.size() * sizeof(uint2));
         COPY(m->d_perm, pk->perm.data(), pk->perm.size() * 4);
         COPY(m->d_items, items.data(), items.size() * sizeof(int4));
@@ -747,44 +781,48 @@ extern "C" int acc_gpu_topk_lowbit(acc_gpu_matrix *m, const float *x_dev, int k,
     return launch_lowbit(m, x_dev, ck / m->split, k, out_dev, nullptr, nullptr, (hipStream_t)stream);
 }
 
-extern "C" int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq, const int32_t *k_host, int ck,
-                                         acc_element *out_dev, int out_stride, void *stream) {
-    if (!m || !x_dev || !k_host || nq < 0 || !out_dev) return fail(ACC_ERR_ARG, "bad batch arguments");
-    if (nq == 0) return ACC_OK;
-    int kmax = 1;
-    for (int q = 0; q < nq; ++q) {
-        if (k_host[q] < 1 || k_host[q] > out_stride) return fail(ACC_ERR_ARG, "each k must be in [1, out_stride]");
-        kmax = std::max(kmax, (int)k_host[q]);
-    }
-    int rc = check_lowbit(m, ck, kmax);
-    if (rc != ACC_OK) return rc;
-    HIP_TRY(hipSetDevice(m->device));
-    hipStream_t st = (hipStream_t)stream;
+// Batched queries with the handle's or a plan's workspace.  k_host: per-query
+// K copied to the workspace (null: already there, as in graph capture).
+static int run_batch(acc_gpu_matrix *m, BatchWs &ws, const float *x_dev, int nq, const int32_t *k_host, int kmax,
+                     int ck, acc_element *out_dev, int out_stride, hipStream_t st) {
     const int M = ck / m->split;
-    if (use_dense(m, nq)) {
-        size_t kcap = (size_t)m->bK_elems;
-        rc = grow(m, &m->d_bK, &kcap, (size_t)nq);
-        m->bK_elems = (int)kcap;
-        if (rc != ACC_OK) return rc;
-        HIP_TRY(hipMemcpyAsync(m->d_bK, k_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
-        return launch_dense(m, x_dev, nq, M, kmax, out_dev, out_stride, nullptr, nullptr, st);
-    }
+    int rc = ws_grow(ws, &ws.K, &ws.K_n, (size_t)nq);
+    if (rc != ACC_OK) return rc;
+    if (k_host) HIP_TRY(hipMemcpyAsync(ws.K, k_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+    if (use_dense(m, nq)) return launch_dense(m, ws, x_dev, nq, M, kmax, out_dev, out_stride, nullptr, nullptr, st);
     if ((size_t)m->cols * (16 + 8) > 65536) {
-        // the query table of a block does not fit in LDS: one query per pass
+        // the packet kernels' query table does not fit in LDS: one query per pass
+        if (!k_host) return fail(ACC_ERR_ARG, "per-query launches need the K values on the host");
         for (int q = 0; q < nq && rc == ACC_OK; ++q)
-            rc = launch_lowbit(m, x_dev + (size_t)q * m->cols, ck / m->split, k_host[q],
-                               out_dev + (size_t)q * out_stride, nullptr, nullptr, st);
+            rc = launch_lowbit(m, x_dev + (size_t)q * m->cols, M, k_host[q], out_dev + (size_t)q * out_stride, nullptr,
+                               nullptr, st);
         return rc;
     }
     const size_t nslots = (size_t)m->nslices * 64;
-    rc = grow(m, &m->d_bS, &m->bS_elems, (size_t)nq * nslots);
-    if (rc == ACC_OK) rc = grow(m, &m->d_bresc, &m->bresc_elems, (size_t)nq * m->split * M);
-    size_t kcap = (size_t)m->bK_elems;
-    if (rc == ACC_OK) rc = grow(m, &m->d_bK, &kcap, (size_t)nq);
-    m->bK_elems = (int)kcap;
+    rc = ws_grow(ws, &ws.S, &ws.S_n, (size_t)nq * nslots);
+    if (rc == ACC_OK) rc = ws_grow(ws, &ws.resc, &ws.resc_n, (size_t)nq * m->split * M);
     if (rc != ACC_OK) return rc;
-    HIP_TRY(hipMemcpyAsync(m->d_bK, k_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
-    return launch_batch(m, x_dev, nq, M, kmax, out_dev, out_stride, nullptr, nullptr, st);
+    return launch_batch(m, ws, x_dev, nq, M, kmax, out_dev, out_stride, nullptr, nullptr, st);
+}
+
+static int check_batch(acc_gpu_matrix *m, const float *x_dev, int nq, const int32_t *k_host, int ck,
+                       acc_element *out_dev, int out_stride, int *kmax) {
+    if (!m || !x_dev || !k_host || nq < 0 || !out_dev) return fail(ACC_ERR_ARG, "bad batch arguments");
+    *kmax = 1;
+    for (int q = 0; q < nq; ++q) {
+        if (k_host[q] < 1 || k_host[q] > out_stride) return fail(ACC_ERR_ARG, "each k must be in [1, out_stride]");
+        *kmax = std::max(*kmax, (int)k_host[q]);
+    }
+    return check_lowbit(m, ck, *kmax);
+}
+
+extern "C" int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq, const int32_t *k_host, int ck,
+                                         acc_element *out_dev, int out_stride, void *stream) {
+    int kmax = 1;
+    int rc = check_batch(m, x_dev, nq, k_host, ck, out_dev, out_stride, &kmax);
+    if (rc != ACC_OK || nq == 0) return rc;
+    HIP_TRY(hipSetDevice(m->device));
+    return run_batch(m, m->ws, x_dev, nq, k_host, kmax, ck, out_dev, out_stride, (hipStream_t)stream);
 }
 
 extern "C" int acc_gpu_scores_lowbit(acc_gpu_matrix *m, const float *x_dev, int32_t *y_dev, void *stream) {
@@ -917,6 +955,127 @@ extern "C" int acc_gpu_top_k_heap(const float *y_dev, int n, int k, acc_element 
     hipFreeAsync(cand, st);
     return rc;
 }
+
+// ---------------------------------------------------------------------------
+// plans: a query launch sequence captured once into a HIP graph
+// ---------------------------------------------------------------------------
+
+struct acc_plan {
+    acc_gpu_matrix *m = nullptr;
+    int kind = 0;
+    const float *x = nullptr;
+    int nq = 0, ck = 0, kmax = 0, out_stride = 0;
+    acc_element *out = nullptr;
+    std::vector<void *> allocs;  // the plan's own batch workspace
+    int64_t bytes = 0;
+    BatchWs ws;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void plan_free(acc_plan *p) {
+    if (!p) return;
+    if (p->m) hipSetDevice(p->m->device);
+    if (p->exec) hipGraphExecDestroy(p->exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    if (p->cap) hipStreamDestroy(p->cap);
+    for (void *q : p->allocs) hipFree(q);
+    delete p;
+}
+
+// The plan's launches; k_host is null under capture (K is already in the
+// plan's workspace).
+static int plan_enqueue(acc_plan *p, const int32_t *k_host, hipStream_t st) {
+    acc_gpu_matrix *m = p->m;
+    switch (p->kind) {
+        case ACC_PLAN_LOWBIT:
+            return launch_lowbit(m, p->x, p->ck / m->split, p->kmax, p->out, nullptr, nullptr, st);
+        case ACC_PLAN_LOWBIT_BATCH:
+            return run_batch(m, p->ws, p->x, p->nq, k_host, p->kmax, p->ck, p->out, p->out_stride, st);
+        case ACC_PLAN_FP32: {
+            int rc = launch_spmv_fp32(m, p->x, m->d_y, st);
+            if (rc != ACC_OK) return rc;
+            return launch_select(m->d_y, m->rows, p->kmax, m->sel_blocks, m->sel_chunk, m->d_sthr, m->d_scnt,
+                                 m->d_scand, p->out, st);
+        }
+        default:
+            return fail(ACC_ERR_ARG, "unknown plan kind");
+    }
+}
+
+extern "C" int acc_gpu_plan_create(acc_gpu_matrix *m, int kind, const float *x_dev, int nq, const int32_t *k_host,
+                                   int ck, acc_element *out_dev, int out_stride, acc_plan **out) {
+    if (!m || !out || !x_dev || !out_dev || !k_host || nq < 1) return fail(ACC_ERR_ARG, "bad plan arguments");
+    int kmax = 1, rc = ACC_OK;
+    if (kind == ACC_PLAN_LOWBIT_BATCH) {
+        rc = check_batch(m, x_dev, nq, k_host, ck, out_dev, out_stride, &kmax);
+    } else if (kind == ACC_PLAN_LOWBIT || kind == ACC_PLAN_FP32) {
+        if (nq != 1) return fail(ACC_ERR_ARG, "single-query plans take nq = 1");
+        kmax = k_host[0];
+        if (kind == ACC_PLAN_LOWBIT) {
+            rc = check_lowbit(m, ck, kmax);
+            if (rc == ACC_OK && kmax < 1) rc = fail(ACC_ERR_ARG, "k must be >= 1");
+        } else if (!m->has_csr || kmax < 1 || kmax > kMaxK) {
+            rc = fail(ACC_ERR_ARG, "FP32 plans need the CSR and k in [1, 960]");
+        }
+    } else {
+        rc = fail(ACC_ERR_ARG, "kind must be ACC_PLAN_LOWBIT, ACC_PLAN_LOWBIT_BATCH or ACC_PLAN_FP32");
+    }
+    if (rc != ACC_OK) return rc;
+    HIP_TRY(hipSetDevice(m->device));
+    acc_plan *p = new acc_plan();
+    p->m = m;
+    p->kind = kind;
+    p->x = x_dev;
+    p->nq = nq;
+    p->ck = ck;
+    p->kmax = kmax;
+    p->out = out_dev;
+    p->out_stride = out_stride;
+    p->ws.allocs = &p->allocs;
+    p->ws.bytes = &p->bytes;
+    const bool prof = m->profiling;
+    m->profiling = false;
+    hipGraph_t g = nullptr;
+    if (hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess) {
+        rc = fail(ACC_ERR_DEVICE, "stream creation failed");
+    } else {
+        // an eager run first: builds what the handle makes lazily, sizes the
+        // plan's workspace, uploads K and raises kernel limits outside capture
+        rc = plan_enqueue(p, k_host, p->cap);
+        if (rc == ACC_OK && hipStreamSynchronize(p->cap) != hipSuccess) rc = fail(ACC_ERR_DEVICE, "eager run failed");
+        if (rc == ACC_OK) {
+            if (hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                rc = fail(ACC_ERR_DEVICE, "stream capture failed to start");
+            } else {
+                rc = plan_enqueue(p, nullptr, p->cap);
+                const hipError_t e = hipStreamEndCapture(p->cap, &g);
+                if (rc == ACC_OK && e != hipSuccess)
+                    rc = fail(ACC_ERR_DEVICE, std::string("stream capture: ") + hipGetErrorString(e));
+            }
+        }
+    }
+    m->profiling = prof;
+    p->graph = g;
+    if (rc == ACC_OK && hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = fail(ACC_ERR_DEVICE, "graph instantiation failed");
+    if (rc != ACC_OK) {
+        plan_free(p);
+        return rc;
+    }
+    *out = p;
+    return ACC_OK;
+}
+
+extern "C" int acc_gpu_plan_launch(acc_plan *p, void *stream) {
+    if (!p) return fail(ACC_ERR_ARG, "null plan");
+    HIP_TRY(hipSetDevice(p->m->device));
+    HIP_TRY(hipGraphLaunch(p->exec, (hipStream_t)stream));
+    return ACC_OK;
+}
+
+extern "C" void acc_gpu_plan_destroy(acc_plan *p) { plan_free(p); }
 
 // ---------------------------------------------------------------------------
 // host-pointer convenience forms and the top_k_spmv drop-in

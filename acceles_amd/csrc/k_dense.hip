@@ -166,7 +166,13 @@ __global__ __launch_bounds__(ACC_WG) void dense_prep(DenseArgs a) {
 // Per slice a wavefront runs 2*NT MFMA chains over the nkb column blocks, the
 // next P blocks of the matrix tiles in flight; the B operands come from the
 // query table in LDS.
-template <int NT>
+template <bool STREAM>
+__device__ __forceinline__ v4i tile_load(const v4i *p) {
+    if (STREAM) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
+template <int NT, bool STREAM>
 __global__ __launch_bounds__(512) void dense_scores(DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int QB = 32 * NT;
@@ -207,7 +213,7 @@ __global__ __launch_bounds__(512) void dense_scores(DenseArgs a) {
     // redundant re-reads) so each lands directly in its slot -- a conditional
     // load would be copied into place after waiting for every outstanding load.
     // The next slice's first DIST blocks are requested before the epilogue.
-    constexpr int RING = 8, DIST = 6;
+    constexpr int RING = 4, DIST = 2;
     v4i pa[RING], pb[RING];
 #pragma unroll
     for (int j = 0; j < RING; ++j) {
@@ -223,8 +229,8 @@ __global__ __launch_bounds__(512) void dense_scores(DenseArgs a) {
 #pragma unroll
         for (int j = 0; j < DIST; ++j) {
             const size_t kj = (size_t)min(j, nkb - 1) * kstr;
-            pa[j] = __builtin_nontemporal_load(B0 + kj);
-            pb[j] = __builtin_nontemporal_load(B0 + kj + 64);
+            pa[j] = tile_load<STREAM>(B0 + kj);
+            pb[j] = tile_load<STREAM>(B0 + kj + 64);
         }
     }
     for (; s < a.nslices; s += stride) {
@@ -241,8 +247,8 @@ __global__ __launch_bounds__(512) void dense_scores(DenseArgs a) {
             for (int j = 0; j < RING; ++j) {
                 const int k = k0 + j;
                 const size_t kn = (size_t)min(k + DIST, nkb - 1) * kstr;
-                pa[(j + DIST) % RING] = __builtin_nontemporal_load(A0 + kn);
-                pb[(j + DIST) % RING] = __builtin_nontemporal_load(A1 + kn);
+                pa[(j + DIST) % RING] = tile_load<STREAM>(A0 + kn);
+                pb[(j + DIST) % RING] = tile_load<STREAM>(A1 + kn);
                 if (k < nkb) {
 #pragma unroll
                     for (int t = 0; t < NT; ++t) {
@@ -259,8 +265,8 @@ __global__ __launch_bounds__(512) void dense_scores(DenseArgs a) {
 #pragma unroll
             for (int j = 0; j < DIST; ++j) {
                 const size_t kj = (size_t)min(j, nkb - 1) * kstr;
-                pa[j] = __builtin_nontemporal_load(B0 + kj);
-                pb[j] = __builtin_nontemporal_load(B0 + kj + 64);
+                pa[j] = tile_load<STREAM>(B0 + kj);
+                pb[j] = tile_load<STREAM>(B0 + kj + 64);
             }
         }
         // D[i][j] of a 32x32 tile: lane j + 32*((i/4)%2), register (i%4) + 4*(i/8);
@@ -551,11 +557,14 @@ __global__ __launch_bounds__(ACC_WG) void batch_topk(DenseArgs a) {
 
 dense_fn get_dense_build() { return dense_build; }
 dense_fn get_dense_prep() { return dense_prep; }
-dense_fn get_dense_scores(int NT) {
+// STREAM: non-temporal tile loads, for matrices larger than the last-level
+// (MALL) cache; smaller ones are read with normal loads and stay resident
+// between batches.
+dense_fn get_dense_scores(int NT, bool stream) {
     switch (NT) {
-        case 1: return dense_scores<1>;
-        case 2: return dense_scores<2>;
-        case 4: return dense_scores<4>;
+        case 1: return stream ? dense_scores<1, true> : dense_scores<1, false>;
+        case 2: return stream ? dense_scores<2, true> : dense_scores<2, false>;
+        case 4: return stream ? dense_scores<4, true> : dense_scores<4, false>;
         default: return nullptr;
     }
 }

@@ -14,60 +14,103 @@ template <int E, bool SIGNED>
 __global__ __launch_bounds__(LB_WG) void lowbit_stage1(LowbitArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_n[LB_WAVES];
+    __shared__ uint2 s_meta[LB_MAX_ITEM];
     float *red = (float *)smem;
     uint64_t *colbuf = (uint64_t *)(smem + 64);
     int8_t *qx = (int8_t *)(smem + 64 + 8 * LB_WAVES * WaveTopM<2>::cap);
-    quantize_query_lds(a.x, a.cols, qx, red);
+    const int4 item = a.items[blockIdx.x];
+    for (int i = threadIdx.x; i < item.z - item.y; i += LB_WG) s_meta[i] = a.slices[item.y + i];
+    quantize_query_lds(a.x, a.cols, qx, red);  // its barriers also publish s_meta
 
     const int wave = threadIdx.x >> 6, lane = lane_id();
-    const int4 item = a.items[blockIdx.x];
     const int p = item.x;
     const uint32_t cmask = (1u << a.col_bits) - 1u;
     WaveTopM<2> col;
     col.init(colbuf + wave * WaveTopM<2>::cap, a.M);
 
-    // ---- stage 1: slices item.y + wave, + LB_WAVES, ...; next group prefetched ----
-    int s = item.y + wave;
-    if (s < item.z) {
-        uint2 sl = a.slices[s];
-        int g0 = 0;
-        uint32_t w[E];
-        load_group<E>(w, a.words + (size_t)sl.x * 64 + lane, min(E, (int)sl.y));
-        uint64_t g = atomic_load_u64(a.thr + p);
-        int32_t acc = 0;
-        for (;;) {
-            const int nd = min(E, (int)sl.y - g0);
-            const bool slice_end = g0 + E >= (int)sl.y;
-            int s2 = s, g02 = g0 + E;
-            uint2 sl2 = sl;
-            if (slice_end) {
-                s2 = s + LB_WAVES;
-                g02 = 0;
-                if (s2 < item.z) sl2 = a.slices[s2];
-            }
-            const bool more = s2 < item.z;
-            uint32_t w2[E];
-            if (more)
-                load_group<E>(w2, a.words + ((size_t)sl2.x + g02) * 64 + lane, min(E, (int)sl2.y - g02));
-            acc = decode_group<E, SIGNED>(w, nd, qx, a.col_bits, a.val_bits, cmask, acc);
-            if (slice_end) {
-                const uint64_t t = col.thr > g ? col.thr : g;
-                const bool maybe = acc >= dkey_score_i32(t);
-                if (__ballot(maybe) != 0) {
-                    const int32_t row = maybe ? a.perm[(size_t)s * 64 + lane] : -1;
-                    const uint64_t key = dkey_i32(acc, row);
-                    col.push(maybe && row >= 0 && key > t, key);
-                }
-                acc = 0;
-                g = atomic_load_u64(a.thr + p);
-            }
-            if (!more) break;
+    // Groups (E dwords per lane) of the wave's slices item.y + wave,
+    // + LB_WAVES, ... stream through a ring of three register slots, two
+    // groups ahead of the decoder.  Every load of the loop is unconditional
+    // (the word array carries E*64 dwords of zero padding past its end;
+    // dwords past a group's length are masked when it is decoded): a load
+    // whose result is merged into a loop variable on only some paths is
+    // copied into place after a wait for all outstanding loads, which would
+    // drain the ring.  A slot records the group's slice, length and whether
+    // it ends the slice.
+    constexpr int RING = 3;
+    uint32_t w[RING][E];
+    int ds[RING], dnd[RING];
+    bool dend[RING], dval[RING];
+    int ls = item.y + wave, lg = 0;  // load cursor
+    uint2 lsl = ls < item.z ? s_meta[ls - item.y] : make_uint2(0u, 0u);
+    auto issue = [&](int slot) {
+        const bool valid = ls < item.z;
+        const size_t base = valid ? ((size_t)lsl.x + lg) * 64 : 0;
 #pragma unroll
-            for (int i = 0; i < E; ++i) w[i] = w2[i];
-            s = s2;
-            g0 = g02;
-            sl = sl2;
+        for (int i = 0; i < E; ++i) w[slot][i] = __builtin_nontemporal_load(a.words + base + (size_t)i * 64 + lane);
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads here: the scheduler would sink them to their use
+        dval[slot] = valid;
+        ds[slot] = ls;
+        dnd[slot] = valid ? min(E, (int)lsl.y - lg) : 0;
+        dend[slot] = valid && lg + E >= (int)lsl.y;
+        if (valid) {
+            if (dend[slot]) {
+                ls += LB_WAVES;
+                lg = 0;
+                lsl = ls < item.z ? s_meta[ls - item.y] : make_uint2(0u, 0u);
+            } else {
+                lg += E;
+            }
         }
+    };
+    // Per step: push the rows requested at the previous slice end, decode the
+    // group in slot j, at a slice end compare with the threshold and request
+    // the rows of the lanes that pass (from a fixed dummy address when none
+    // does, so the load is unconditional), then request the threshold for the
+    // next slice end and finally the group two steps ahead.  Every load is
+    // consumed one step after it is issued, so no wait reaches the ring's
+    // newest groups.
+    uint64_t g = atomic_load_u64(a.thr + p);
+    issue(0);
+    issue(1);
+    int32_t acc = 0, pacc = 0, prow = -1;
+    uint64_t pt = 0;
+    bool pend = false, pmaybe = false;
+    bool done = false;
+    while (!done) {
+#pragma unroll
+        for (int j = 0; j < RING; ++j) {
+            if (done) break;
+            if (!dval[j]) {
+                done = true;
+                break;
+            }
+            if (pend) {
+                const uint64_t key = dkey_i32(pacc, prow);
+                col.push(pmaybe && prow >= 0 && key > pt, key);
+            }
+            const int nd = dnd[j];
+            uint32_t wm[E];
+#pragma unroll
+            for (int i = 0; i < E; ++i) wm[i] = i < nd ? w[j][i] : 0u;
+            acc = decode_group<E, SIGNED>(wm, nd, qx, a.col_bits, a.val_bits, cmask, acc);
+            bool maybe = false;
+            if (dend[j]) {
+                pt = col.thr > g ? col.thr : g;
+                maybe = acc >= dkey_score_i32(pt);
+                pacc = acc;
+                acc = 0;
+            }
+            pend = __ballot(maybe) != 0;
+            pmaybe = maybe;
+            prow = a.perm[pend ? (size_t)ds[j] * 64 + lane : (size_t)lane];
+            g = atomic_load_u64(a.thr + p);
+            issue((j + 2) % RING);
+        }
+    }
+    if (pend) {
+        const uint64_t key = dkey_i32(pacc, prow);
+        col.push(pmaybe && prow >= 0 && key > pt, key);
     }
     col.compact();
     if (lane == 0) s_n[wave] = col.n;

@@ -27,6 +27,7 @@
 
 #define LB_WAVES 8
 #define LB_WG (LB_WAVES * 64)
+#define LB_MAX_ITEM 1024  // slices per stage-1 work item (their metadata sits in LDS)
 
 // Quantize the FP32 query into LDS: q = clip(rint(127 * (x / max|x|)), -128, 127)
 // (quantify_vector with B=8, symmetric; float32 division and product, half-even).
@@ -354,7 +355,7 @@ batch_fn get_batch_final(int R);
 typedef void (*dense_fn)(DenseArgs);
 dense_fn get_dense_build();                                    // k_dense.hip
 dense_fn get_dense_prep();
-dense_fn get_dense_scores(int NT);
+dense_fn get_dense_scores(int NT, bool stream);
 dense_fn get_dense_select();
 dense_fn get_batch_rescore();
 dense_fn get_batch_topk(int R);

@@ -276,6 +276,28 @@ int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq,
 #define ACC_BATCH_DENSE 2
 int acc_gpu_set_batch_mode(acc_gpu_matrix *m, int mode, int min_dense_batch);
 
+/* Plans: one query launch sequence captured into a HIP graph and replayed
+ * (the reference re-enqueues its kernel and buffer migrations for every query,
+ * host_spmv_ucsr.cpp:1084-1180; a plan does that setup once).  The plan reads
+ * x_dev and writes out_dev -- fixed buffers the caller refills between
+ * launches -- and keeps its own batch workspace; the handle must outlive it.
+ * Creation runs the sequence once eagerly (results land in out_dev), then
+ * captures it.  Kinds:
+ *   ACC_PLAN_LOWBIT        acc_gpu_topk_lowbit, nq = 1, k = k_host[0];
+ *   ACC_PLAN_LOWBIT_BATCH  acc_gpu_topk_lowbit_batch (per-query K);
+ *   ACC_PLAN_FP32          acc_gpu_topk_fp32, nq = 1, k = k_host[0].
+ * Launches of plans and calls on one handle must not overlap in time (they
+ * share its selection state): issue them on one stream. */
+typedef struct acc_plan acc_plan;
+#define ACC_PLAN_LOWBIT 1
+#define ACC_PLAN_LOWBIT_BATCH 2
+#define ACC_PLAN_FP32 3
+int acc_gpu_plan_create(acc_gpu_matrix *m, int kind, const float *x_dev, int nq,
+                        const int32_t *k_host, int ck, acc_element *out_dev, int out_stride,
+                        acc_plan **plan);
+int acc_gpu_plan_launch(acc_plan *plan, void *stream);
+void acc_gpu_plan_destroy(acc_plan *plan);
+
 /* Stage outputs, for parity checks.
  *   spmv_fp32:      y_dev[rows] FP32 scores (the Top-K path's order)
  *   scores_lowbit:  y_dev[rows] INT32 stage-1 scores, original row order

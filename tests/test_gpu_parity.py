@@ -322,3 +322,45 @@ def test_embedding_scale_parity_and_recall(acc, gpu):
         full.append([len(set(i[:K]) & set(exact[:K])) / K for K in KS])
     rec = np.mean(full, axis=0)
     assert rec[-1] >= 0.99, rec
+
+
+# --------------------------------------------------------------------------
+# plans (HIP graphs): every replay equals the eager call on the same queries
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("kind,nq,mode", [("lowbit", 1, 0), ("fp32", 1, 0), ("batch", 40, 1), ("batch", 40, 2),
+                                          ("batch", 130, 2)])
+def test_plan_replays_match_eager(acc, gpu, kind, nq, mode):
+    csr, pk, g = make(acc, 20000, 500, (40, 100), 8, 32, seed=77)
+    if mode:
+        g.set_batch_mode(mode)
+    K, CK = 50, 512
+    x = empty(acc, nq * 500, np.float32)
+    out = empty(acc, nq * K * 2, np.int32)
+    ref = empty(acc, nq * K * 2, np.int32)
+    other = empty(acc, 7 * K * 2, np.int32)
+    ks = np.array([[1, 8, 16, 32, 50][i % 5] for i in range(nq)], np.int32) if kind == "batch" else np.array([K], np.int32)
+    X0 = acc.generate_queries(nq, 500, seed=1000)
+    acc.lib().acc_memcpy_h2d(x.ptr, X0.ctypes.data, X0.nbytes)
+    pkind = {"lowbit": acc.PLAN_LOWBIT, "fp32": acc.PLAN_FP32, "batch": acc.PLAN_LOWBIT_BATCH}[kind]
+    plan = acc.Plan(g, pkind, x, ks, CK, out, K)
+    Xo = dev(acc, acc.generate_queries(7, 500, seed=5))
+    for rep in range(4):
+        X = acc.generate_queries(nq, 500, seed=2000 + rep)
+        acc.lib().acc_memcpy_h2d(x.ptr, X.ctypes.data, X.nbytes)
+        plan.launch()
+        acc.synchronize()
+        # eager work on the same handle between replays (another batch shape)
+        g.topk_lowbit_batch(Xo, np.full(7, 9, np.int32), CK, other, K)
+        acc.synchronize()
+        if kind == "lowbit":
+            g.topk_lowbit(x, K, CK, ref)
+        elif kind == "fp32":
+            g.topk_fp32(x, K, ref)
+        else:
+            g.topk_lowbit_batch(x, ks, CK, ref, K)
+        acc.synchronize()
+        a, b = out.numpy().reshape(nq, K, 2), ref.numpy().reshape(nq, K, 2)
+        for qi in range(nq):
+            np.testing.assert_array_equal(a[qi, :ks[qi]], b[qi, :ks[qi]], err_msg=f"replay {rep} query {qi}")
+    plan.close()
