@@ -29,7 +29,7 @@ ERRORS = {
 }
 DEG_UNIFORM, DEG_GAMMA = 0, 1
 VAL_ABSNORMAL, VAL_NORMAL, VAL_UNIFORM = 0, 1, 2
-BATCH_AUTO, BATCH_SPARSE, BATCH_DENSE = 0, 1, 2
+BATCH_AUTO, BATCH_SPARSE, BATCH_DENSE, BATCH_SINGLE = 0, 1, 2, 3
 PLAN_LOWBIT, PLAN_LOWBIT_BATCH, PLAN_FP32 = 1, 2, 3
 
 
@@ -57,6 +57,11 @@ class PackedInfo(C.Structure):
                 ("nnz", C.c_int64), ("nnz_kept", C.c_int64), ("nslices", C.c_int64),
                 ("stream_bytes", C.c_int64), ("perm_bytes", C.c_int64), ("bytes_per_nnz", C.c_double),
                 ("scale", C.c_float)]
+
+
+class TunePoint(C.Structure):
+    _fields_ = [("nq", C.c_int32), ("layout", C.c_int32), ("us_single", C.c_float), ("us_sparse", C.c_float),
+                ("us_dense", C.c_float)]
 
 
 class GpuInfo(C.Structure):
@@ -121,6 +126,7 @@ SIGNATURES = [
     ("acc_gpu_matrix_destroy", None, [_P]),
     ("acc_gpu_set_profiling", C.c_int, [_P, C.c_int]),
     ("acc_gpu_set_batch_mode", C.c_int, [_P, C.c_int, C.c_int]),
+    ("acc_gpu_autotune", C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.POINTER(C.c_int)]),
     ("acc_gpu_plan_create", C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, C.POINTER(_P)]),
     ("acc_gpu_plan_launch", C.c_int, [_P, _P]),
     ("acc_gpu_plan_destroy", None, [_P]),
@@ -527,6 +533,13 @@ class GpuMatrix:
     def set_batch_mode(self, mode, min_dense_batch=0):
         """BATCH_AUTO, BATCH_SPARSE or BATCH_DENSE scoring for topk_lowbit_batch."""
         _check(lib().acc_gpu_set_batch_mode(self._h, int(mode), int(min_dense_batch)))
+
+    def autotune(self, ck, k, max_batch):
+        """acc_gpu_autotune: per batch size, the fastest layout (list of TunePoint)."""
+        pts = (TunePoint * 16)()
+        n = C.c_int()
+        _check(lib().acc_gpu_autotune(self._h, int(ck), int(k), int(max_batch), pts, 16, C.byref(n)))
+        return [pts[i] for i in range(n.value)]
 
     def set_profiling(self, enable=True):
         _check(lib().acc_gpu_set_profiling(self._h, int(enable)))

@@ -93,7 +93,12 @@ struct acc_gpu_matrix {
     std::string dense_why;           // why not, otherwise
     bool has_dense = false;
     int batch_mode = ACC_BATCH_AUTO;
-    int dense_min_nq = 16;           // ACC_BATCH_AUTO: dense tiles from this batch size on
+    int dense_min_nq = 16;           // ACC_BATCH_AUTO, untuned: dense tiles from this batch size on
+    // ACC_BATCH_AUTO after acc_gpu_autotune: the layout measured fastest at
+    // tune_nq[i], used for batches of tune_nq[i] .. tune_nq[i+1]-1 queries
+    int ntune = 0;
+    int tune_nq[16] = {0};
+    int tune_layout[16] = {0};
     int cols_pad = 0, nkb = 0, u8off = 0;
     int8_t *d_tiles = nullptr;
     int64_t dense_bytes = 0;
@@ -382,9 +387,16 @@ int build_dense(acc_gpu_matrix *m) {
     return ACC_OK;
 }
 
-bool use_dense(const acc_gpu_matrix *m, int nq) {
-    if (m->batch_mode == ACC_BATCH_DENSE) return true;
-    return m->batch_mode == ACC_BATCH_AUTO && m->dense_ok && nq >= m->dense_min_nq;
+// Layout of a batch of nq queries: ACC_BATCH_SINGLE, _SPARSE or _DENSE.
+int batch_layout(const acc_gpu_matrix *m, int nq) {
+    if (m->batch_mode != ACC_BATCH_AUTO) return m->batch_mode;
+    if (m->ntune > 0) {
+        int pick = m->tune_layout[0];
+        for (int i = 0; i < m->ntune; ++i)
+            if (m->tune_nq[i] <= nq) pick = m->tune_layout[i];
+        return pick;
+    }
+    return m->dense_ok && nq >= m->dense_min_nq ? ACC_BATCH_DENSE : ACC_BATCH_SPARSE;
 }
 
 // Batched low-bit queries on the dense tiles: dense_prep, dense_scores,
@@ -753,18 +765,21 @@ extern "C" void acc_gpu_matrix_destroy(acc_gpu_matrix *m) {
 
 extern "C" int acc_gpu_set_batch_mode(acc_gpu_matrix *m, int mode, int min_dense_batch) {
     if (!m) return fail(ACC_ERR_ARG, "null matrix");
-    if (mode != ACC_BATCH_AUTO && mode != ACC_BATCH_SPARSE && mode != ACC_BATCH_DENSE)
-        return fail(ACC_ERR_ARG, "mode must be ACC_BATCH_AUTO, ACC_BATCH_SPARSE or ACC_BATCH_DENSE");
+    if (mode != ACC_BATCH_AUTO && mode != ACC_BATCH_SPARSE && mode != ACC_BATCH_DENSE && mode != ACC_BATCH_SINGLE)
+        return fail(ACC_ERR_ARG, "mode must be ACC_BATCH_AUTO, _SINGLE, _SPARSE or _DENSE");
     if (min_dense_batch < 0) return fail(ACC_ERR_ARG, "min_dense_batch must be >= 0");
     if (mode != ACC_BATCH_SPARSE && !m->has_packed) return fail(ACC_ERR_ARG, "matrix was uploaded without low-bit packets");
     if (mode == ACC_BATCH_DENSE && !m->dense_ok) return fail(ACC_ERR_ARG, "dense tiles unavailable: " + m->dense_why);
     HIP_TRY(hipSetDevice(m->device));
-    if (mode != ACC_BATCH_SPARSE && m->dense_ok) {
+    if ((mode == ACC_BATCH_DENSE || mode == ACC_BATCH_AUTO) && m->dense_ok) {
         int rc = build_dense(m);
         if (rc != ACC_OK) return rc;
     }
     m->batch_mode = mode;
-    if (min_dense_batch > 0) m->dense_min_nq = min_dense_batch;
+    if (min_dense_batch > 0) {
+        m->dense_min_nq = min_dense_batch;
+        m->ntune = 0;  // an explicit crossover replaces a tuned table
+    }
     return ACC_OK;
 }
 
@@ -782,22 +797,25 @@ extern "C" int acc_gpu_topk_lowbit(acc_gpu_matrix *m, const float *x_dev, int k,
 }
 
 // Batched queries with the handle's or a plan's workspace.  k_host: per-query
-// K copied to the workspace (null: already there, as in graph capture).
-static int run_batch(acc_gpu_matrix *m, BatchWs &ws, const float *x_dev, int nq, const int32_t *k_host, int kmax,
-                     int ck, acc_element *out_dev, int out_stride, hipStream_t st) {
+// K on the host; upload_k copies it into the workspace (false under graph
+// capture, where the eager run already did).
+static int run_batch(acc_gpu_matrix *m, BatchWs &ws, const float *x_dev, int nq, const int32_t *k_host, bool upload_k,
+                     int kmax, int ck, acc_element *out_dev, int out_stride, hipStream_t st) {
     const int M = ck / m->split;
-    int rc = ws_grow(ws, &ws.K, &ws.K_n, (size_t)nq);
-    if (rc != ACC_OK) return rc;
-    if (k_host) HIP_TRY(hipMemcpyAsync(ws.K, k_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
-    if (use_dense(m, nq)) return launch_dense(m, ws, x_dev, nq, M, kmax, out_dev, out_stride, nullptr, nullptr, st);
-    if ((size_t)m->cols * (16 + 8) > 65536) {
-        // the packet kernels' query table does not fit in LDS: one query per pass
-        if (!k_host) return fail(ACC_ERR_ARG, "per-query launches need the K values on the host");
+    int layout = batch_layout(m, nq);
+    // the packet kernels' query table of a 16-query block must fit in LDS
+    if (layout == ACC_BATCH_SPARSE && (size_t)m->cols * (16 + 8) > 65536) layout = ACC_BATCH_SINGLE;
+    if (layout == ACC_BATCH_SINGLE) {
+        int rc = ACC_OK;
         for (int q = 0; q < nq && rc == ACC_OK; ++q)
             rc = launch_lowbit(m, x_dev + (size_t)q * m->cols, M, k_host[q], out_dev + (size_t)q * out_stride, nullptr,
                                nullptr, st);
         return rc;
     }
+    int rc = ws_grow(ws, &ws.K, &ws.K_n, (size_t)nq);
+    if (rc != ACC_OK) return rc;
+    if (upload_k) HIP_TRY(hipMemcpyAsync(ws.K, k_host, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+    if (layout == ACC_BATCH_DENSE) return launch_dense(m, ws, x_dev, nq, M, kmax, out_dev, out_stride, nullptr, nullptr, st);
     const size_t nslots = (size_t)m->nslices * 64;
     rc = ws_grow(ws, &ws.S, &ws.S_n, (size_t)nq * nslots);
     if (rc == ACC_OK) rc = ws_grow(ws, &ws.resc, &ws.resc_n, (size_t)nq * m->split * M);
@@ -822,7 +840,7 @@ extern "C" int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, 
     int rc = check_batch(m, x_dev, nq, k_host, ck, out_dev, out_stride, &kmax);
     if (rc != ACC_OK || nq == 0) return rc;
     HIP_TRY(hipSetDevice(m->device));
-    return run_batch(m, m->ws, x_dev, nq, k_host, kmax, ck, out_dev, out_stride, (hipStream_t)stream);
+    return run_batch(m, m->ws, x_dev, nq, k_host, true, kmax, ck, out_dev, out_stride, (hipStream_t)stream);
 }
 
 extern "C" int acc_gpu_scores_lowbit(acc_gpu_matrix *m, const float *x_dev, int32_t *y_dev, void *stream) {
@@ -957,6 +975,89 @@ extern "C" int acc_gpu_top_k_heap(const float *y_dev, int n, int k, acc_element 
 }
 
 // ---------------------------------------------------------------------------
+// autotuner for ACC_BATCH_AUTO
+// ---------------------------------------------------------------------------
+
+extern "C" int acc_gpu_autotune(acc_gpu_matrix *m, int ck, int k, int max_batch, acc_tune_point *points, int cap,
+                                int *count) {
+    int rc = check_lowbit(m, ck, k);
+    if (rc != ACC_OK) return rc;
+    if (k < 1 || max_batch < 1 || max_batch > 4096 || cap < 0 || (cap > 0 && !points))
+        return fail(ACC_ERR_ARG, "k >= 1, max_batch in [1, 4096] and room for the points are required");
+    HIP_TRY(hipSetDevice(m->device));
+    if (m->dense_ok && (rc = build_dense(m)) != ACC_OK) return rc;
+    std::vector<float> xh((size_t)max_batch * m->cols);
+    acc_generate_queries(max_batch, m->cols, ACC_VAL_NORMAL, 0, 0x7E57ull, xh.data());
+    float *x = nullptr;
+    acc_element *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<void *> tmp;
+    int64_t tmp_bytes = 0;
+    size_t xn = 0, on = 0;
+    rc = grow_in(tmp, tmp_bytes, &x, &xn, xh.size());
+    if (rc == ACC_OK) rc = grow_in(tmp, tmp_bytes, &out, &on, (size_t)max_batch * k);
+    if (rc == ACC_OK && (hipMemcpy(x, xh.data(), xh.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                         hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+                         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(ACC_ERR_DEVICE, "autotune setup failed");
+    const int saved_mode = m->batch_mode;
+    const bool prof = m->profiling;
+    m->profiling = false;
+    m->ntune = 0;
+    int n = 0;
+    for (int nq = 1; rc == ACC_OK && nq <= max_batch && m->ntune < 16; nq = nq < max_batch && nq * 2 > max_batch ? max_batch : nq * 2) {
+        std::vector<int32_t> ks((size_t)nq, k);
+        float us[3] = {-1.f, -1.f, -1.f};
+        const int layouts[3] = {ACC_BATCH_SINGLE, ACC_BATCH_SPARSE, ACC_BATCH_DENSE};
+        for (int li = 0; li < 3 && rc == ACC_OK; ++li) {
+            if (layouts[li] == ACC_BATCH_DENSE && !m->dense_ok) continue;
+            if (layouts[li] == ACC_BATCH_SPARSE && (size_t)m->cols * (16 + 8) > 65536) continue;
+            m->batch_mode = layouts[li];
+            float best = 1e30f;
+            for (int r = 0; r < 4 && rc == ACC_OK; ++r) {  // the first run warms up
+                hipEventRecord(e0, st);
+                rc = run_batch(m, m->ws, x, nq, ks.data(), true, k, ck, out, k, st);
+                hipEventRecord(e1, st);
+                float ms = 0.f;
+                if (rc == ACC_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    rc = fail(ACC_ERR_DEVICE, "autotune timing failed");
+                if (r > 0) best = std::min(best, ms * 1000.f);
+            }
+            us[li] = best;
+        }
+        int pick = ACC_BATCH_SINGLE;
+        float t = us[0];
+        for (int li = 1; li < 3; ++li)
+            if (us[li] >= 0.f && us[li] < t) {
+                t = us[li];
+                pick = layouts[li];
+            }
+        m->tune_nq[m->ntune] = nq;
+        m->tune_layout[m->ntune] = pick;
+        ++m->ntune;
+        if (n < cap) {
+            points[n].nq = nq;
+            points[n].layout = pick;
+            points[n].us_single = us[0];
+            points[n].us_sparse = us[1];
+            points[n].us_dense = us[2];
+            ++n;
+        }
+        if (nq == max_batch) break;
+    }
+    m->profiling = prof;
+    m->batch_mode = rc == ACC_OK ? ACC_BATCH_AUTO : saved_mode;
+    if (rc != ACC_OK) m->ntune = 0;
+    if (count) *count = n;
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    for (void *q : tmp) hipFree(q);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
 // plans: a query launch sequence captured once into a HIP graph
 // ---------------------------------------------------------------------------
 
@@ -966,6 +1067,7 @@ struct acc_plan {
     const float *x = nullptr;
     int nq = 0, ck = 0, kmax = 0, out_stride = 0;
     acc_element *out = nullptr;
+    std::vector<int32_t> k;      // per-query K (host copy)
     std::vector<void *> allocs;  // the plan's own batch workspace
     int64_t bytes = 0;
     BatchWs ws;
@@ -984,15 +1086,15 @@ static void plan_free(acc_plan *p) {
     delete p;
 }
 
-// The plan's launches; k_host is null under capture (K is already in the
+// The plan's launches; upload_k is false under capture (K is already in the
 // plan's workspace).
-static int plan_enqueue(acc_plan *p, const int32_t *k_host, hipStream_t st) {
+static int plan_enqueue(acc_plan *p, bool upload_k, hipStream_t st) {
     acc_gpu_matrix *m = p->m;
     switch (p->kind) {
         case ACC_PLAN_LOWBIT:
             return launch_lowbit(m, p->x, p->ck / m->split, p->kmax, p->out, nullptr, nullptr, st);
         case ACC_PLAN_LOWBIT_BATCH:
-            return run_batch(m, p->ws, p->x, p->nq, k_host, p->kmax, p->ck, p->out, p->out_stride, st);
+            return run_batch(m, p->ws, p->x, p->nq, p->k.data(), upload_k, p->kmax, p->ck, p->out, p->out_stride, st);
         case ACC_PLAN_FP32: {
             int rc = launch_spmv_fp32(m, p->x, m->d_y, st);
             if (rc != ACC_OK) return rc;
@@ -1033,6 +1135,7 @@ extern "C" int acc_gpu_plan_create(acc_gpu_matrix *m, int kind, const float *x_d
     p->kmax = kmax;
     p->out = out_dev;
     p->out_stride = out_stride;
+    p->k.assign(k_host, k_host + nq);
     p->ws.allocs = &p->allocs;
     p->ws.bytes = &p->bytes;
     const bool prof = m->profiling;
@@ -1043,13 +1146,13 @@ extern "C" int acc_gpu_plan_create(acc_gpu_matrix *m, int kind, const float *x_d
     } else {
         // an eager run first: builds what the handle makes lazily, sizes the
         // plan's workspace, uploads K and raises kernel limits outside capture
-        rc = plan_enqueue(p, k_host, p->cap);
+        rc = plan_enqueue(p, true, p->cap);
         if (rc == ACC_OK && hipStreamSynchronize(p->cap) != hipSuccess) rc = fail(ACC_ERR_DEVICE, "eager run failed");
         if (rc == ACC_OK) {
             if (hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                 rc = fail(ACC_ERR_DEVICE, "stream capture failed to start");
             } else {
-                rc = plan_enqueue(p, nullptr, p->cap);
+                rc = plan_enqueue(p, false, p->cap);
                 const hipError_t e = hipStreamEndCapture(p->cap, &g);
                 if (rc == ACC_OK && e != hipSuccess)
                     rc = fail(ACC_ERR_DEVICE, std::string("stream capture: ") + hipGetErrorString(e));

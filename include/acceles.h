@@ -274,7 +274,23 @@ int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq,
 #define ACC_BATCH_AUTO 0
 #define ACC_BATCH_SPARSE 1
 #define ACC_BATCH_DENSE 2
+#define ACC_BATCH_SINGLE 3 /* one single-query pass per query */
 int acc_gpu_set_batch_mode(acc_gpu_matrix *m, int mode, int min_dense_batch);
+
+/* Autotuner for ACC_BATCH_AUTO (no counterpart in the reference, whose
+ * partition count and CK are fixed at build time, types.hpp:54,81): times one
+ * query per pass, the packet batch and the dense batch (where the matrix
+ * allows it) on synthetic queries at batch sizes 1, 2, 4, ... up to max_batch,
+ * and keeps the fastest layout per batch size in the handle (builds the dense
+ * tiles).  points[i] receives each measured size (up to cap entries; *count
+ * gets the number written); times are per batch in microseconds, -1 where a
+ * layout does not apply.  Results do not depend on the layout. */
+typedef struct {
+    int32_t nq, layout;   /* layout chosen: ACC_BATCH_SINGLE / _SPARSE / _DENSE */
+    float us_single, us_sparse, us_dense;
+} acc_tune_point;
+int acc_gpu_autotune(acc_gpu_matrix *m, int ck, int k, int max_batch, acc_tune_point *points,
+                     int cap, int *count);
 
 /* Plans: one query launch sequence captured into a HIP graph and replayed
  * (the reference re-enqueues its kernel and buffer migrations for every query,

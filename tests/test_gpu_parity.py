@@ -364,3 +364,17 @@ def test_plan_replays_match_eager(acc, gpu, kind, nq, mode):
         for qi in range(nq):
             np.testing.assert_array_equal(a[qi, :ks[qi]], b[qi, :ks[qi]], err_msg=f"replay {rep} query {qi}")
     plan.close()
+
+
+def test_autotune_keeps_results(acc, gpu):
+    """The tuned layout choice per batch size changes speed, not results."""
+    csr, pk, g = make(acc, 20000, 500, (40, 100), 8, 32, seed=91)
+    pts = g.autotune(512, 50, 64)
+    assert [p.nq for p in pts] == [1, 2, 4, 8, 16, 32, 64]
+    for p in pts:
+        assert p.layout in (acc.BATCH_SINGLE, acc.BATCH_SPARSE, acc.BATCH_DENSE)
+        assert p.us_single > 0 and p.us_sparse > 0 and p.us_dense > 0
+    for nq in (1, 3, 20, 64):
+        X = acc.generate_queries(nq, 500, seed=nq + 90)
+        ks = np.array([[1, 8, 16, 32, 50][i % 5] for i in range(nq)], np.int32)
+        check_batch(acc, g, csr, 8, False, X, ks, 512, 32, out_stride=50)
