@@ -59,6 +59,20 @@ class PackedInfo(C.Structure):
                 ("scale", C.c_float)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("device", C.c_int), ("memory_budget", C.c_int64), ("max_batch", C.c_int), ("slots", C.c_int),
+                ("ck", C.c_int), ("max_pending", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("refused", C.c_int64), ("batches", C.c_int64),
+                ("mean_batch", C.c_double), ("p50_us", C.c_double), ("p99_us", C.c_double), ("max_us", C.c_double),
+                ("qps", C.c_double), ("device_bytes", C.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class TunePoint(C.Structure):
     _fields_ = [("nq", C.c_int32), ("layout", C.c_int32), ("us_single", C.c_float), ("us_sparse", C.c_float),
                 ("us_dense", C.c_float)]
@@ -127,6 +141,14 @@ SIGNATURES = [
     ("acc_gpu_set_profiling", C.c_int, [_P, C.c_int]),
     ("acc_gpu_set_batch_mode", C.c_int, [_P, C.c_int, C.c_int]),
     ("acc_gpu_autotune", C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.POINTER(C.c_int)]),
+    ("acc_server_create", C.c_int, [_P, C.POINTER(_P)]),
+    ("acc_server_add_matrix", C.c_int, [_P, _P, _P, C.POINTER(C.c_int)]),
+    ("acc_server_submit", C.c_int, [_P, C.c_int, _P, C.c_int, C.POINTER(C.c_uint64)]),
+    ("acc_server_wait", C.c_int, [_P, C.c_uint64, _P, C.c_int]),
+    ("acc_server_stats", C.c_int, [_P, _P]),
+    ("acc_server_reset_stats", C.c_int, [_P]),
+    ("acc_server_bench_poisson", C.c_int, [_P, C.c_double, C.c_double, C.c_int, C.c_uint64, _P]),
+    ("acc_server_destroy", None, [_P]),
     ("acc_gpu_plan_create", C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, C.POINTER(_P)]),
     ("acc_gpu_plan_launch", C.c_int, [_P, _P]),
     ("acc_gpu_plan_destroy", None, [_P]),
@@ -503,6 +525,59 @@ class Plan:
     def close(self):
         if self._h:
             lib().acc_gpu_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop over resident matrices with continuous batching (acc_server_*)."""
+
+    def __init__(self, device=0, memory_budget=0, max_batch=256, slots=4, ck=512, max_pending=0):
+        o = ServerOpts(device, memory_budget, max_batch, slots, ck, max_pending)
+        h = _P()
+        _check(lib().acc_server_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self.cols = []
+
+    def add_matrix(self, csr, packed):
+        mid = C.c_int()
+        _check(lib().acc_server_add_matrix(self._h, csr._h, packed._h, C.byref(mid)))
+        self.cols.append(csr.cols)
+        return mid.value
+
+    def submit(self, matrix_id, x, k):
+        x = _f32(x)
+        t = C.c_uint64()
+        _check(lib().acc_server_submit(self._h, int(matrix_id), x.ctypes.data, int(k), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket, k, timeout_ms=-1):
+        out = np.zeros(k, ELEMENT_DTYPE)
+        _check(lib().acc_server_wait(self._h, C.c_uint64(ticket), out.ctypes.data, int(timeout_ms)))
+        return out
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().acc_server_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self):
+        _check(lib().acc_server_reset_stats(self._h))
+
+    def bench_poisson(self, rate_qps, seconds, kmax=100, seed=1):
+        st = ServerStats()
+        _check(lib().acc_server_bench_poisson(self._h, float(rate_qps), float(seconds), int(kmax), int(seed),
+                                              C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().acc_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

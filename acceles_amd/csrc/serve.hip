@@ -1,0 +1,470 @@
+// serve.hip -- request loop over device-resident matrices with continuous
+// batching (host code only).
+//
+// The reference answers one query at a time: reset, upload the query, run,
+// read back (host_spmv_ucsr.cpp:1084-1180; cpu_mkl_baseline.cpp:254-272).
+// Here a worker thread keeps every registered matrix resident in device
+// memory and turns the queue of arriving requests into batches:
+//   * a matrix has at most one batch in flight (its handle's workspace and
+//     selection state are reused between batches); batches of different
+//     matrices run concurrently on separate streams ("slots");
+//   * whenever a slot is free, the matrix whose oldest waiting request is the
+//     oldest gets the slot and up to max_batch of its waiting requests, each
+//     with its own K -- requests that arrive while a batch runs are admitted
+//     into the next one as soon as the slot frees (continuous batching);
+//   * matrices are admitted against a device-memory budget that includes the
+//     workspace of a full batch; requests against the queue capacity.
+// Results are those of acc_gpu_topk_lowbit_batch, which do not depend on the
+// batch a query lands in.
+#include "internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <random>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+using acc::fail;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+constexpr int kMaxSlots = 8;
+constexpr int kMaxK = 960;
+
+struct Request {
+    int matrix = 0;
+    int k = 0;
+    bool keep = true;  // false: results are discarded (load generator)
+    std::vector<float> x;
+    std::vector<acc_element> out;
+    Clock::time_point t0, t1;
+    bool done = false;
+    int status = ACC_OK;
+};
+
+struct Matrix {
+    acc_gpu_matrix *m = nullptr;
+    int cols = 0;
+    std::deque<uint64_t> pending;
+    bool inflight = false;
+    int64_t bytes = 0;
+};
+
+struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    float *x_pin = nullptr;        // max_batch x max_cols
+    acc_element *o_pin = nullptr;  // max_batch x kMaxK
+    float *x_dev = nullptr;
+    acc_element *o_dev = nullptr;
+    std::vector<uint64_t> ids;
+    std::vector<int32_t> ks;
+    int matrix = -1;
+    int stride = 0;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct acc_server {
+    acc_server_opts o{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::unordered_map<uint64_t, Request> reqs;
+    std::vector<Matrix> mats;
+    Slot slots[kMaxSlots];
+    std::thread worker;
+    bool stop = false;
+    uint64_t next_id = 1;
+    int64_t used = 0;
+    int max_cols = 8192;
+    int64_t npending = 0;
+    // statistics since the last reset
+    std::vector<double> lat_us;
+    int64_t submitted = 0, completed = 0, refused = 0, batches = 0, batched = 0;
+    Clock::time_point t_reset = Clock::now();
+};
+
+namespace {
+
+// Completed batches: copy results out, release the slot and the matrix.
+void reap(acc_server *s) {
+    for (int i = 0; i < s->o.slots; ++i) {
+        Slot &sl = s->slots[i];
+        if (!sl.busy) continue;
+        const hipError_t e = hipEventQuery(sl.ev);
+        if (e == hipErrorNotReady) continue;
+        const int status = e == hipSuccess ? ACC_OK : ACC_ERR_DEVICE;
+        const auto now = Clock::now();
+        for (size_t j = 0; j < sl.ids.size(); ++j) {
+            auto it = s->reqs.find(sl.ids[j]);
+            if (it == s->reqs.end()) continue;
+            Request &r = it->second;
+            if (r.keep && status == ACC_OK) r.out.assign(sl.o_pin + j * sl.stride, sl.o_pin + j * sl.stride + r.k);
+            r.t1 = now;
+            r.done = true;
+            r.status = status;
+            s->lat_us.push_back(std::chrono::duration<double, std::micro>(r.t1 - r.t0).count());
+            ++s->completed;
+            if (!r.keep) s->reqs.erase(it);
+        }
+        s->mats[(size_t)sl.matrix].inflight = false;
+        sl.busy = false;
+        sl.ids.clear();
+    }
+}
+
+// Gives free slots to the matrices whose oldest waiting request is oldest.
+void launch(acc_server *s) {
+    for (int i = 0; i < s->o.slots; ++i) {
+        Slot &sl = s->slots[i];
+        if (sl.busy) continue;
+        int best = -1;
+        Clock::time_point oldest = Clock::time_point::max();
+        for (size_t mi = 0; mi < s->mats.size(); ++mi) {
+            Matrix &mx = s->mats[mi];
+            if (!mx.m || mx.inflight || mx.pending.empty()) continue;
+            const Clock::time_point t = s->reqs[mx.pending.front()].t0;
+            if (t < oldest) {
+                oldest = t;
+                best = (int)mi;
+            }
+        }
+        if (best < 0) return;
+        Matrix &mx = s->mats[(size_t)best];
+        const int nq = (int)std::min<size_t>(mx.pending.size(), (size_t)s->o.max_batch);
+        sl.ids.assign(mx.pending.begin(), mx.pending.begin() + nq);
+        mx.pending.erase(mx.pending.begin(), mx.pending.begin() + nq);
+        s->npending -= nq;
+        sl.ks.resize((size_t)nq);
+        int kmax = 1;
+        for (int j = 0; j < nq; ++j) {
+            const Request &r = s->reqs[sl.ids[(size_t)j]];
+            std::memcpy(sl.x_pin + (size_t)j * mx.cols, r.x.data(), (size_t)mx.cols * 4);
+            sl.ks[(size_t)j] = r.k;
+            kmax = std::max(kmax, r.k);
+        }
+        sl.stride = kmax;
+        sl.matrix = best;
+        int rc = hipMemcpyAsync(sl.x_dev, sl.x_pin, (size_t)nq * mx.cols * 4, hipMemcpyHostToDevice, sl.st) == hipSuccess
+                     ? ACC_OK : ACC_ERR_DEVICE;
+        if (rc == ACC_OK)
+            rc = acc_gpu_topk_lowbit_batch(mx.m, sl.x_dev, nq, sl.ks.data(), s->o.ck, sl.o_dev, kmax, sl.st);
+        if (rc == ACC_OK &&
+            hipMemcpyAsync(sl.o_pin, sl.o_dev, (size_t)nq * kmax * sizeof(acc_element), hipMemcpyDeviceToHost, sl.st) != hipSuccess)
+            rc = ACC_ERR_DEVICE;
+        if (rc == ACC_OK && hipEventRecord(sl.ev, sl.st) != hipSuccess) rc = ACC_ERR_DEVICE;
+        if (rc != ACC_OK) {  // fail these requests; the slot stays free
+            for (uint64_t id : sl.ids) {
+                Request &r = s->reqs[id];
+                r.done = true;
+                r.status = rc;
+                ++s->completed;
+            }
+            sl.ids.clear();
+            s->cv_done.notify_all();
+            continue;
+        }
+        mx.inflight = true;
+        sl.busy = true;
+        ++s->batches;
+        s->batched += nq;
+    }
+}
+
+void worker_main(acc_server *s) {
+    hipSetDevice(s->o.device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (!s->stop) {
+        const int64_t before = s->completed;
+        reap(s);
+        if (s->completed != before) s->cv_done.notify_all();
+        launch(s);
+        bool busy = false;
+        for (int i = 0; i < s->o.slots; ++i) busy |= s->slots[i].busy;
+        if (busy)
+            s->cv_work.wait_for(lk, std::chrono::microseconds(20));  // poll the in-flight batches
+        else
+            s->cv_work.wait(lk, [s] { return s->stop || s->npending > 0; });
+    }
+}
+
+void free_slots(acc_server *s) {
+    for (Slot &sl : s->slots) {
+        if (sl.ev) hipEventDestroy(sl.ev);
+        if (sl.st) hipStreamDestroy(sl.st);
+        if (sl.x_pin) hipHostFree(sl.x_pin);
+        if (sl.o_pin) hipHostFree(sl.o_pin);
+        if (sl.x_dev) hipFree(sl.x_dev);
+        if (sl.o_dev) hipFree(sl.o_dev);
+        sl = Slot();
+    }
+}
+
+}  // namespace
+
+extern "C" int acc_server_create(const acc_server_opts *opts, acc_server **out) {
+    if (!opts || !out) return fail(ACC_ERR_ARG, "null argument");
+    acc_server_opts o = *opts;
+    if (o.max_batch == 0) o.max_batch = 256;
+    if (o.slots == 0) o.slots = 4;
+    if (o.ck == 0) o.ck = 512;
+    if (o.max_pending == 0) o.max_pending = 1 << 16;
+    if (o.max_batch < 1 || o.max_batch > 1024 || o.slots < 1 || o.slots > kMaxSlots || o.ck < 1 || o.max_pending < 1 ||
+        o.memory_budget < 0)
+        return fail(ACC_ERR_ARG, "max_batch in [1, 1024], slots in [1, 8], ck >= 1, max_pending >= 1, budget >= 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(ACC_ERR_NODEVICE, "no GPU visible");
+    if (o.device < 0 || o.device >= ndev) return fail(ACC_ERR_ARG, "bad device index");
+    if (hipSetDevice(o.device) != hipSuccess) return fail(ACC_ERR_DEVICE, "hipSetDevice failed");
+    if (o.memory_budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(ACC_ERR_DEVICE, "hipMemGetInfo failed");
+        o.memory_budget = (int64_t)(fr / 10 * 9);
+    }
+    acc_server *s = new acc_server();
+    s->o = o;
+    int rc = ACC_OK;
+    for (int i = 0; i < o.slots && rc == ACC_OK; ++i) {
+        Slot &sl = s->slots[i];
+        const size_t xb = (size_t)o.max_batch * s->max_cols * 4, ob = (size_t)o.max_batch * kMaxK * sizeof(acc_element);
+        if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc((void **)&sl.x_pin, xb, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&sl.o_pin, ob, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc((void **)&sl.x_dev, xb) != hipSuccess || hipMalloc((void **)&sl.o_dev, ob) != hipSuccess)
+            rc = fail(ACC_ERR_NOMEM, "server slot allocation failed");
+        s->used += (int64_t)(xb + ob);
+    }
+    if (rc == ACC_OK && s->used > o.memory_budget) rc = fail(ACC_ERR_BUDGET, "memory budget below the slot buffers");
+    if (rc != ACC_OK) {
+        free_slots(s);
+        delete s;
+        return rc;
+    }
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return ACC_OK;
+}
+
+extern "C" int acc_server_add_matrix(acc_server *s, const acc_csr *csr, const acc_packed *pk, int *matrix_id) {
+    if (!s || !csr || !pk || !matrix_id) return fail(ACC_ERR_ARG, "a server, the FP32 CSR and its packets are required");
+    if (csr->cols > s->max_cols) return fail(ACC_ERR_ARG, "cols must be <= 8192");
+    if (hipSetDevice(s->o.device) != hipSuccess) return fail(ACC_ERR_DEVICE, "hipSetDevice failed");
+    acc_gpu_matrix *m = nullptr;
+    int rc = acc_gpu_upload(csr, pk, s->o.device, &m);
+    if (rc != ACC_OK) return rc;
+    // admission: the resident matrix plus the workspace of a full batch and of
+    // a small one (dense tiles, if the matrix can use them, are built here)
+    acc_gpu_info_t info;
+    acc_gpu_matrix_info(m, &info);
+    int64_t have = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        have = s->o.memory_budget - s->used;
+    }
+    if (info.device_bytes > have) {
+        acc_gpu_matrix_destroy(m);
+        return fail(ACC_ERR_BUDGET, "matrix does not fit in the server's memory budget");
+    }
+    if (info.dense_ok) {
+        const int64_t nsl = (info.rows + 63) / 64 + info.split;
+        const int64_t dense = nsl * 2 * (((int64_t)info.cols + 31) / 32) * 1024;
+        if (info.device_bytes + dense > have) rc = acc_gpu_set_batch_mode(m, ACC_BATCH_SPARSE, 0);
+        else rc = acc_gpu_set_batch_mode(m, ACC_BATCH_AUTO, 0);
+    }
+    const int warm[2] = {std::min(15, s->o.max_batch), s->o.max_batch};
+    float *x = nullptr;
+    acc_element *o = nullptr;
+    if (rc == ACC_OK && (hipMalloc((void **)&x, (size_t)s->o.max_batch * csr->cols * 4) != hipSuccess ||
+                         hipMalloc((void **)&o, (size_t)s->o.max_batch * sizeof(acc_element)) != hipSuccess))
+        rc = fail(ACC_ERR_NOMEM, "warm-up buffers");
+    if (rc == ACC_OK && hipMemset(x, 0, (size_t)s->o.max_batch * csr->cols * 4) != hipSuccess) rc = ACC_ERR_DEVICE;
+    for (int w = 0; w < 2 && rc == ACC_OK; ++w) {
+        std::vector<int32_t> ks((size_t)warm[w], 1);
+        rc = acc_gpu_topk_lowbit_batch(m, x, warm[w], ks.data(), s->o.ck, o, 1, nullptr);
+        if (rc == ACC_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(ACC_ERR_DEVICE, "warm-up batch failed");
+    }
+    if (x) hipFree(x);
+    if (o) hipFree(o);
+    if (rc == ACC_OK) acc_gpu_matrix_info(m, &info);
+    if (rc == ACC_OK && info.device_bytes > have) rc = fail(ACC_ERR_BUDGET, "matrix and its batch workspace exceed the budget");
+    if (rc != ACC_OK) {
+        acc_gpu_matrix_destroy(m);
+        return rc;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    Matrix mx;
+    mx.m = m;
+    mx.cols = csr->cols;
+    mx.bytes = info.device_bytes;
+    s->used += mx.bytes;
+    s->mats.push_back(std::move(mx));
+    *matrix_id = (int)s->mats.size() - 1;
+    return ACC_OK;
+}
+
+static int submit(acc_server *s, int matrix_id, const float *x, int k, bool keep, Clock::time_point t0,
+                  uint64_t *ticket) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (matrix_id < 0 || matrix_id >= (int)s->mats.size() || !s->mats[(size_t)matrix_id].m)
+        return fail(ACC_ERR_ARG, "unknown matrix");
+    if (k < 1 || k > std::min(kMaxK, s->o.ck)) return fail(ACC_ERR_ARG, "k must be in [1, min(960, ck)]");
+    if (s->npending >= s->o.max_pending) {
+        ++s->refused;
+        return fail(ACC_ERR_BUDGET, "request queue full");
+    }
+    Matrix &mx = s->mats[(size_t)matrix_id];
+    const uint64_t id = s->next_id++;
+    Request &r = s->reqs[id];
+    r.matrix = matrix_id;
+    r.k = k;
+    r.keep = keep;
+    r.x.assign(x, x + mx.cols);
+    r.t0 = t0;
+    mx.pending.push_back(id);
+    ++s->npending;
+    ++s->submitted;
+    if (ticket) *ticket = id;
+    s->cv_work.notify_one();
+    return ACC_OK;
+}
+
+extern "C" int acc_server_submit(acc_server *s, int matrix_id, const float *x, int k, uint64_t *ticket) {
+    if (!s || !x || !ticket) return fail(ACC_ERR_ARG, "null argument");
+    return submit(s, matrix_id, x, k, true, Clock::now(), ticket);
+}
+
+extern "C" int acc_server_wait(acc_server *s, uint64_t ticket, acc_element *out, int timeout_ms) {
+    if (!s) return fail(ACC_ERR_ARG, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(ACC_ERR_ARG, "unknown ticket");
+    auto ready = [&] { return s->reqs[ticket].done; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return fail(ACC_ERR_BUDGET, "timed out");
+    Request &r = s->reqs[ticket];
+    const int status = r.status;
+    if (status == ACC_OK && out) std::copy(r.out.begin(), r.out.end(), out);
+    s->reqs.erase(ticket);
+    return status == ACC_OK ? ACC_OK : fail(status, "request failed on the device");
+}
+
+static void fill_stats(acc_server *s, acc_server_stats_t *st) {
+    std::memset(st, 0, sizeof(*st));
+    st->submitted = s->submitted;
+    st->completed = s->completed;
+    st->refused = s->refused;
+    st->batches = s->batches;
+    st->mean_batch = s->batches ? (double)s->batched / (double)s->batches : 0.0;
+    std::vector<double> l = s->lat_us;
+    if (!l.empty()) {
+        std::sort(l.begin(), l.end());
+        auto pct = [&](double q) { return l[std::min(l.size() - 1, (size_t)(q * (double)(l.size() - 1) + 0.5))]; };
+        st->p50_us = pct(0.50);
+        st->p99_us = pct(0.99);
+        st->max_us = l.back();
+    }
+    const double secs = std::chrono::duration<double>(Clock::now() - s->t_reset).count();
+    st->qps = secs > 0 ? (double)s->completed / secs : 0.0;
+    st->device_bytes = s->used;
+}
+
+extern "C" int acc_server_stats(acc_server *s, acc_server_stats_t *st) {
+    if (!s || !st) return fail(ACC_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    fill_stats(s, st);
+    return ACC_OK;
+}
+
+extern "C" int acc_server_reset_stats(acc_server *s) {
+    if (!s) return fail(ACC_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lat_us.clear();
+    s->submitted = s->completed = s->refused = s->batches = s->batched = 0;
+    s->t_reset = Clock::now();
+    return ACC_OK;
+}
+
+extern "C" int acc_server_bench_poisson(acc_server *s, double rate_qps, double seconds, int kmax, uint64_t seed,
+                                        acc_server_stats_t *st) {
+    if (!s || !st || rate_qps <= 0 || seconds <= 0 || kmax < 1) return fail(ACC_ERR_ARG, "bad load parameters");
+    int nmat = 0, cols_max = 0;
+    std::vector<int> cols;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        nmat = (int)s->mats.size();
+        for (auto &m : s->mats) cols.push_back(m.cols);
+    }
+    if (nmat == 0) return fail(ACC_ERR_ARG, "no matrices");
+    for (int c : cols) cols_max = std::max(cols_max, c);
+    // a pool of N(0,1) queries per matrix; arrivals: exponential gaps
+    constexpr int kPool = 256;
+    std::vector<std::vector<float>> pool((size_t)nmat);
+    for (int i = 0; i < nmat; ++i) {
+        pool[(size_t)i].resize((size_t)kPool * cols[(size_t)i]);
+        acc_generate_queries(kPool, cols[(size_t)i], ACC_VAL_NORMAL, 0, seed + (uint64_t)i, pool[(size_t)i].data());
+    }
+    std::mt19937_64 rng(seed);
+    std::exponential_distribution<double> gap(rate_qps);
+    std::uniform_int_distribution<int> pick_m(0, nmat - 1), pick_q(0, kPool - 1), pick_k(1, std::min(kmax, std::min(kMaxK, s->o.ck)));
+    acc_server_reset_stats(s);
+    const Clock::time_point start = Clock::now();
+    double t = 0.0;
+    int64_t refused = 0;
+    int rc = ACC_OK;
+    while (t < seconds && rc == ACC_OK) {
+        t += gap(rng);
+        const Clock::time_point due = start + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(t));
+        while (Clock::now() < due) {
+        }
+        const int mi = pick_m(rng);
+        const float *x = pool[(size_t)mi].data() + (size_t)pick_q(rng) * cols[(size_t)mi];
+        uint64_t id = 0;
+        // latency counts from the scheduled arrival (no coordinated omission)
+        const int r = submit(s, mi, x, pick_k(rng), false, due, &id);
+        if (r == ACC_ERR_BUDGET) ++refused;
+        else if (r != ACC_OK) rc = r;
+    }
+    // drain
+    for (;;) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        bool idle = s->npending == 0;
+        for (int i = 0; i < s->o.slots; ++i) idle = idle && !s->slots[i].busy;
+        if (idle) {
+            fill_stats(s, st);
+            const double secs = std::chrono::duration<double>(Clock::now() - start).count();
+            st->qps = secs > 0 ? (double)s->completed / secs : 0.0;
+            break;
+        }
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(1));
+    }
+    (void)refused;
+    return rc;
+}
+
+extern "C" void acc_server_destroy(acc_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    hipSetDevice(s->o.device);
+    hipDeviceSynchronize();
+    for (auto &m : s->mats)
+        if (m.m) acc_gpu_matrix_destroy(m.m);
+    free_slots(s);
+    delete s;
+}

@@ -314,6 +314,50 @@ int acc_gpu_plan_create(acc_gpu_matrix *m, int kind, const float *x_dev, int nq,
 int acc_gpu_plan_launch(acc_plan *plan, void *stream);
 void acc_gpu_plan_destroy(acc_plan *plan);
 
+/* ---- Serving: resident matrices, continuous batching ----------------
+ * No counterpart in the reference, which answers one query at a time
+ * (host_spmv_ucsr.cpp:1084-1180, cpu_mkl_baseline.cpp:254-272); for callers
+ * that issue many concurrent queries.  A worker thread keeps the matrices
+ * resident and forms batches from the waiting requests of each matrix (up to
+ * max_batch, each request with its own K); batches of different matrices
+ * run concurrently on up to `slots` streams, and requests that arrive while a
+ * batch runs join the next batch of their matrix as soon as it can start.
+ * Matrices are admitted against memory_budget (resident data plus the
+ * workspace of a full batch: ACC_ERR_BUDGET otherwise), requests against
+ * max_pending.  A result does not depend on the batch it lands in. */
+typedef struct acc_server acc_server;
+typedef struct {
+    int device;
+    int64_t memory_budget; /* device bytes for matrices, workspaces, staging; 0 = 90% of free */
+    int max_batch;         /* queries per batch, 1..1024 (0 = 256)                 */
+    int slots;             /* batches in flight, 1..8 (0 = 4)                       */
+    int ck;                /* CK of every query (0 = 512)                           */
+    int max_pending;       /* waiting requests before submissions are refused (0 = 65536) */
+} acc_server_opts;
+typedef struct {
+    int64_t submitted, completed, refused, batches;
+    double mean_batch;         /* queries per batch                                 */
+    double p50_us, p99_us, max_us; /* latency from arrival to completion            */
+    double qps;                /* completed per second since the last reset         */
+    int64_t device_bytes;      /* held by the server                                */
+} acc_server_stats_t;
+int acc_server_create(const acc_server_opts *opts, acc_server **out);
+/* Uploads the matrix (FP32 CSR for rescoring + packets), runs a warm-up
+ * batch and checks the budget. */
+int acc_server_add_matrix(acc_server *s, const acc_csr *csr, const acc_packed *pk, int *matrix_id);
+/* Queues one query (x: cols floats, copied); 1 <= k <= min(960, ck). */
+int acc_server_submit(acc_server *s, int matrix_id, const float *x, int k, uint64_t *ticket);
+/* Waits for a ticket (timeout_ms < 0: no limit) and copies its k results. */
+int acc_server_wait(acc_server *s, uint64_t ticket, acc_element *out, int timeout_ms);
+int acc_server_stats(acc_server *s, acc_server_stats_t *stats);
+int acc_server_reset_stats(acc_server *s);
+/* Open-loop load: Poisson arrivals at rate_qps for `seconds`, each query
+ * N(0,1) to a uniformly chosen matrix with K uniform in [1, kmax]; latency is
+ * measured from the scheduled arrival.  Returns the run's statistics. */
+int acc_server_bench_poisson(acc_server *s, double rate_qps, double seconds, int kmax, uint64_t seed,
+                             acc_server_stats_t *stats);
+void acc_server_destroy(acc_server *s);
+
 /* Stage outputs, for parity checks.
  *   spmv_fp32:      y_dev[rows] FP32 scores (the Top-K path's order)
  *   scores_lowbit:  y_dev[rows] INT32 stage-1 scores, original row order

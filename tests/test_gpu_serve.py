@@ -1,0 +1,100 @@
+"""Serving loop (acc_server_*) on the GPU: results alone equal results in the
+mix, against the oracle, memory and queue admission, the load generator."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def matrices(acc):
+    specs = [  # rows, cols, degree, kind, bits, split
+        (20000, 500, (40, 100), acc.DEG_UNIFORM, 8, 32),
+        (30000, 900, (30, 90), acc.DEG_UNIFORM, 8, 16),
+        (40000, 1024, (0.5, 40.0), acc.DEG_GAMMA, 4, 32),
+    ]
+    out = []
+    for i, (r, c, d, kind, bits, split) in enumerate(specs):
+        csr = acc.CSRMatrix.generate(r, c, deg=d, deg_kind=kind, seed=500 + i)
+        out.append((csr, csr.pack(bits=bits, split=split), bits, split))
+    return out
+
+
+def test_server_alone_equals_mix_and_oracle(acc, gpu):
+    mats = matrices(acc)
+    srv = acc.Server(max_batch=64, slots=3, ck=512)
+    ids = [srv.add_matrix(csr, pk) for csr, pk, _, _ in mats]
+    rng = np.random.default_rng(3)
+    probes = []
+    for j in range(12):
+        mi = j % 3
+        x = acc.generate_queries(1, mats[mi][0].cols, seed=900 + j)[0]
+        probes.append((mi, x, [1, 8, 16, 32, 50, 75, 100][j % 7]))
+    alone = []
+    for mi, x, k in probes:
+        alone.append(srv.wait(srv.submit(ids[mi], x, k), k))
+    # the same probes among 400 other requests of all matrices and K
+    tickets, order = [], rng.permutation(400 + len(probes))
+    for t in order:
+        if t < len(probes):
+            mi, x, k = probes[t]
+            tickets.append(("p", t, srv.submit(ids[mi], x, k), k))
+        else:
+            mi = int(rng.integers(0, 3))
+            x = rng.standard_normal(mats[mi][0].cols).astype(np.float32)
+            k = int(rng.integers(1, 101))
+            tickets.append(("o", t, srv.submit(ids[mi], x, k), k))
+    mixed = {}
+    for kind, t, tk, k in tickets:
+        res = srv.wait(tk, k)
+        if kind == "p":
+            mixed[t] = res
+    st = srv.stats()
+    assert st["batches"] < st["completed"], "requests were batched"
+    for j, (mi, x, k) in enumerate(probes):
+        np.testing.assert_array_equal(mixed[j], alone[j], err_msg=f"probe {j}")
+        csr, pk, bits, split = mats[mi]
+        ptr, idx, val = csr.arrays()
+        q, _ = O.quantize(val, bits, False)
+        gm = acc.GpuMatrix(csr, pk)  # G of the server's copy of this matrix
+        _, oi, ov = O.two_stage(ptr, idx, val, q, x, csr.cols, gm.info.group, split, 512, k)
+        np.testing.assert_array_equal(alone[j]["index"], oi, err_msg=f"probe {j}")
+        np.testing.assert_array_equal(alone[j]["value"], ov, err_msg=f"probe {j}")
+    srv.close()
+
+
+def test_server_admission(acc, gpu):
+    csr = acc.CSRMatrix.generate(20000, 500, deg=(40, 100), seed=7)
+    pk = csr.pack(bits=8, split=32)
+    # slot buffers alone: 2 x 64 x (8192*4 + 960*8) bytes
+    small = acc.Server(memory_budget=12 << 20, max_batch=64, slots=2)
+    with pytest.raises(acc.AccelesError, match="budget"):
+        small.add_matrix(csr, pk)
+    small.close()
+    srv = acc.Server(max_batch=8, slots=1, max_pending=4)
+    mid = srv.add_matrix(csr, pk)
+    x = acc.generate_queries(1, 500, seed=1)[0]
+    refused = 0
+    tickets = []
+    for _ in range(64):
+        try:
+            tickets.append(srv.submit(mid, x, 10))
+        except acc.AccelesError as e:
+            assert "queue full" in str(e)
+            refused += 1
+    for t in tickets:
+        srv.wait(t, 10)
+    assert refused > 0 and srv.stats()["refused"] == refused
+    srv.close()
+
+
+def test_server_poisson_load(acc, gpu):
+    mats = matrices(acc)
+    srv = acc.Server(max_batch=128, slots=3)
+    for csr, pk, _, _ in mats:
+        srv.add_matrix(csr, pk)
+    st = srv.bench_poisson(5000, 0.5, kmax=100, seed=11)
+    assert st["completed"] == st["submitted"] > 1000
+    assert 0 < st["p50_us"] <= st["p99_us"] <= st["max_us"]
+    srv.close()
