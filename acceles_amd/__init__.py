@@ -61,7 +61,7 @@ class PackedInfo(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("memory_budget", C.c_int64), ("max_batch", C.c_int), ("slots", C.c_int),
-                ("ck", C.c_int), ("max_pending", C.c_int)]
+                ("ck", C.c_int), ("max_pending", C.c_int), ("autotune", C.c_int)]
 
 
 class ServerStats(C.Structure):
@@ -537,8 +537,8 @@ class Plan:
 class Server:
     """Request loop over resident matrices with continuous batching (acc_server_*)."""
 
-    def __init__(self, device=0, memory_budget=0, max_batch=256, slots=4, ck=512, max_pending=0):
-        o = ServerOpts(device, memory_budget, max_batch, slots, ck, max_pending)
+    def __init__(self, device=0, memory_budget=0, max_batch=256, slots=4, ck=512, max_pending=0, autotune=False):
+        o = ServerOpts(device, memory_budget, max_batch, slots, ck, max_pending, int(autotune))
         h = _P()
         _check(lib().acc_server_create(C.byref(o), C.byref(h)))
         self._h = h

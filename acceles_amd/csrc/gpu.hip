@@ -985,7 +985,9 @@ extern "C" int acc_gpu_autotune(acc_gpu_matrix *m, int ck, int k, int max_batch,
     if (k < 1 || max_batch < 1 || max_batch > 4096 || cap < 0 || (cap > 0 && !points))
         return fail(ACC_ERR_ARG, "k >= 1, max_batch in [1, 4096] and room for the points are required");
     HIP_TRY(hipSetDevice(m->device));
-    if (m->dense_ok && (rc = build_dense(m)) != ACC_OK) return rc;
+    // the dense layout competes unless the caller pinned the packet or single-query layout
+    const bool try_dense = m->dense_ok && m->batch_mode != ACC_BATCH_SPARSE && m->batch_mode != ACC_BATCH_SINGLE;
+    if (try_dense && (rc = build_dense(m)) != ACC_OK) return rc;
     std::vector<float> xh((size_t)max_batch * m->cols);
     acc_generate_queries(max_batch, m->cols, ACC_VAL_NORMAL, 0, 0x7E57ull, xh.data());
     float *x = nullptr;
@@ -1011,7 +1013,7 @@ extern "C" int acc_gpu_autotune(acc_gpu_matrix *m, int ck, int k, int max_batch,
         float us[3] = {-1.f, -1.f, -1.f};
         const int layouts[3] = {ACC_BATCH_SINGLE, ACC_BATCH_SPARSE, ACC_BATCH_DENSE};
         for (int li = 0; li < 3 && rc == ACC_OK; ++li) {
-            if (layouts[li] == ACC_BATCH_DENSE && !m->dense_ok) continue;
+            if (layouts[li] == ACC_BATCH_DENSE && !try_dense) continue;
             if (layouts[li] == ACC_BATCH_SPARSE && (size_t)m->cols * (16 + 8) > 65536) continue;
             m->batch_mode = layouts[li];
             float best = 1e30f;

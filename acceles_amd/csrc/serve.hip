@@ -296,6 +296,8 @@ extern "C" int acc_server_add_matrix(acc_server *s, const acc_csr *csr, const ac
     }
     if (x) hipFree(x);
     if (o) hipFree(o);
+    if (rc == ACC_OK && s->o.autotune)
+        rc = acc_gpu_autotune(m, s->o.ck, std::min(100, std::min(kMaxK, s->o.ck)), s->o.max_batch, nullptr, 0, nullptr);
     if (rc == ACC_OK) acc_gpu_matrix_info(m, &info);
     if (rc == ACC_OK && info.device_bytes > have) rc = fail(ACC_ERR_BUDGET, "matrix and its batch workspace exceed the budget");
     if (rc != ACC_OK) {

@@ -52,6 +52,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-mode", choices=["auto", "packets", "dense"], default="auto",
                     help="scoring layout of the batched path")
+    ap.add_argument("--workload", choices=["embedding", "large-single", "batched", "serve"], default="embedding",
+                    help="embedding: 400,000x500 batched (default line); large-single: 20,000,000x1024 single "
+                         "queries; batched: 2,000,000x900, 256 queries of mixed K per graph launch; serve: three "
+                         "resident matrices under Poisson arrivals")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="serve: seconds per offered rate")
+    ap.add_argument("--deg-dist", choices=["uniform", "gamma"], default="uniform",
+                    help="large-single: row lengths U[10,30] or Gamma(0.5, 40) (mean 20)")
     return ap.parse_args()
 
 
@@ -80,6 +87,155 @@ def timed(stream, fn, steps, warmup):
     return time.perf_counter() - t0
 
 
+def exact_topk(g, acc, x_dev, rows, K, stream):
+    """FP64 exact Top-K rows (score desc, row asc) of one device query."""
+    y64 = acc.DeviceBuffer((rows,), np.float64)
+    g.spmv_fp64(x_dev, y64, stream)
+    stream.synchronize()
+    yy = y64.numpy()
+    top = np.argpartition(-yy, K)[:K + 1]
+    return top[np.lexsort((top, -yy[top]))][:K]
+
+
+def precision_at(got, exact):
+    return [len(set(got[:k]) & set(exact[:k])) / k for k in KS]
+
+
+def line_common(a, metric, value, unit, ms, dtype, workload, extra_cfg):
+    return {"metric": metric, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+            "config": dict({"workload": workload, "parallelism": "single GPU"}, **extra_cfg)}
+
+
+def bench_large_single(a, acc, stream):
+    """20,000,000 x 1024, average degree 20, 8-bit, SPLIT=64, CK=1024, K=100: one query per step."""
+    rows, cols, K, CK = 20_000_000, 1024, 100, 1024
+    t0 = time.perf_counter()
+    if a.deg_dist == "uniform":
+        csr = acc.CSRMatrix.generate(rows, cols, deg=(10, 30), l2norm=False, seed=0xACCE1E5 + 2)
+    else:
+        csr = acc.CSRMatrix.generate(rows, cols, deg=(0.5, 40.0), deg_kind=acc.DEG_GAMMA, l2norm=False,
+                                     seed=0xACCE1E5 + 2)
+    pk = csr.pack(bits=8, split=64)
+    g = acc.GpuMatrix(csr, pk, device=0)
+    log(f"[bench] setup {time.perf_counter() - t0:.1f}s: nnz={csr.nnz} packed={pk.info.stream_bytes / 1e9:.3f}GB "
+        f"({pk.info.bytes_per_nnz:.3f} B/nnz) fp32={g.info.csr_bytes / 1e9:.3f}GB")
+    nq = max(a.steps, 8)
+    X = acc.generate_queries(nq, cols, seed=0xACCE1E5 + 3)
+    Xd = acc.DeviceBuffer.from_numpy(X)
+    out = acc.DeviceBuffer((nq, K, 2), np.int32)
+    xs = [Xd.at(i * cols) for i in range(nq)]
+    os_ = [out.at(i * K * 2) for i in range(nq)]
+    i = [0]
+
+    def step():
+        g.topk_lowbit(xs[i[0] % nq], K, CK, os_[i[0] % nq], stream)
+        i[0] += 1
+    dt = timed(stream, step, a.steps, a.warmup)
+    qps = a.steps / dt
+    g.set_profiling(True)
+    lb, fp = [], []
+    for j in range(10):
+        g.topk_lowbit(xs[j % nq], K, CK, os_[0], stream)
+        lb.append(g.stage_times())
+    for j in range(4):
+        g.topk_fp32(xs[j % nq], K, os_[0], stream)
+        fp.append(g.stage_times())
+    g.set_profiling(False)
+    lb_ms, fp_ms = np.mean(lb, axis=0), np.mean(fp, axis=0)
+    s1_bytes = pk.info.stream_bytes + cols * 4
+    s1_gbs = s1_bytes / (lb_ms[0] * 1e-3) / 1e9
+    fqps = 1e3 / float(np.sum(fp_ms))
+    for j in range(nq):
+        g.topk_lowbit(xs[j], K, CK, os_[j], stream)
+    res = out.numpy()
+    prec = [precision_at(res[j, :, 1], exact_topk(g, acc, xs[j], rows, K, stream)) for j in range(min(nq, 8))]
+    precision = dict(zip(map(str, KS), np.round(np.mean(prec, axis=0), 5).tolist()))
+    log(f"[bench] single query: {qps:.0f} q/s; stage1 {lb_ms[0] * 1e3:.1f}us ({s1_gbs:.0f} GB/s), merge "
+        f"{lb_ms[1] * 1e3:.1f}us, final {lb_ms[2] * 1e3:.1f}us; fp32 CSR Top-K {fqps:.0f} q/s; precision {precision}")
+    line = line_common(a, "single-query low-bit Top-100 SpMV queries/s", qps, "queries/s", dt * 1e3 / a.steps, "u8",
+                       f"single-query low-bit Top-100: {rows}x{cols}, row nnz {a.deg_dist} mean 20, |N(0,1)|, 8-bit "
+                       f"packets, SPLIT=64, CK=1024", {"rows": rows, "cols": cols, "nnz": csr.nnz})
+    line.update({
+        "roofline": {"bound": "hbm", "kernel": "lowbit_stage1", "achieved": round(s1_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(s1_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(s1_bytes), "avg_launch_us": round(float(lb_ms[0]) * 1e3, 2)},
+        "cpu_baseline": None,
+        "lowbit_launch_us": [round(float(v) * 1e3, 2) for v in lb_ms],
+        "fp32_launch_us": [round(float(v) * 1e3, 2) for v in fp_ms], "fp32_csr_gpu_qps": round(fqps, 2),
+        "bytes_per_nnz_packed": round(pk.info.bytes_per_nnz, 4), "precision_at_k": precision,
+    })
+    print(json.dumps(line), flush=True)
+
+
+def bench_batched(a, acc, stream):
+    """2,000,000 x 900, row nnz U[30,90], L2-normalised; 256 queries per step with K drawn from
+    {1,8,16,32,50,75,100}; CK=512, SPLIT=32; autotuned layout; one HIP-graph launch per step."""
+    rows, cols, B, CK = 2_000_000, 900, 256, 512
+    t0 = time.perf_counter()
+    csr = acc.CSRMatrix.generate(rows, cols, deg=(30, 90), seed=0xACCE1E5 + 4)
+    pk = csr.pack(bits=a.bits, split=32)
+    g = acc.GpuMatrix(csr, pk, device=0)
+    log(f"[bench] setup {time.perf_counter() - t0:.1f}s: nnz={csr.nnz} packed={pk.info.stream_bytes / 1e6:.0f}MB")
+    pts = g.autotune(CK, 100, B)
+    log("[bench] autotune: " + ", ".join(f"nq={p.nq}:{['auto', 'packets', 'dense', 'single'][p.layout]}"
+                                         f"({p.us_single:.0f}/{p.us_sparse:.0f}/{p.us_dense:.0f}us)" for p in pts))
+    rng = np.random.default_rng(0xACCE1E5 + 5)
+    ks = rng.choice(KS, B).astype(np.int32)
+    X = acc.generate_queries(B, cols, seed=0xACCE1E5 + 6)
+    Xd = acc.DeviceBuffer.from_numpy(X)
+    out = acc.DeviceBuffer((B, 100, 2), np.int32)
+    plan = acc.Plan(g, acc.PLAN_LOWBIT_BATCH, Xd, ks, CK, out, 100)
+    dt = timed(stream, lambda: plan.launch(stream), a.steps, a.warmup)
+    qps = a.steps * B / dt
+    res = out.numpy()
+    prec = []
+    for j in range(0, B, 32):
+        k = int(ks[j])
+        ex = exact_topk(g, acc, Xd.at(j * cols), rows, 100, stream)
+        prec.append(len(set(res[j, :k, 1]) & set(ex[:k])) / k)
+    log(f"[bench] batched: {qps:.0f} q/s ({dt * 1e3 / a.steps:.3f} ms per {B}-query graph launch); "
+        f"mean precision@K {np.mean(prec):.4f}")
+    line = line_common(a, "batched low-bit Top-K queries/s (mixed K)", qps, "queries/s", dt * 1e3 / a.steps,
+                       "u8" if a.bits == 8 else f"u{a.bits}",
+                       f"low-bit two-stage, batched: {rows}x{cols}, row nnz U[30,90], L2-normalised, {a.bits}-bit, "
+                       f"SPLIT=32, CK=512, {B} queries per HIP-graph launch, K from {{1,8,16,32,50,75,100}}",
+                       {"rows": rows, "cols": cols, "nnz": csr.nnz, "queries_per_step": B})
+    line.update({"roofline": None, "cpu_baseline": None, "mean_precision_at_k": round(float(np.mean(prec)), 5),
+                 "autotune": [{"nq": p.nq, "layout": int(p.layout), "us_single": round(p.us_single, 1),
+                               "us_packets": round(p.us_sparse, 1), "us_dense": round(p.us_dense, 1)} for p in pts]})
+    plan.close()
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(a, acc, stream):
+    """Three resident matrices (1,000,000x500 deg 60, 5,000,000x900 deg 40, 10,000,000x1024 deg 20;
+    |N(0,1)|, 8-bit), Poisson arrivals at 2,000 / 10,000 / 50,000 q/s, random matrix, K uniform in [1,100]."""
+    specs = [(1_000_000, 500, 60), (5_000_000, 900, 40), (10_000_000, 1024, 20)]
+    t0 = time.perf_counter()
+    srv = acc.Server(max_batch=256, slots=3, ck=512, autotune=True)
+    for i, (r, c, d) in enumerate(specs):
+        csr = acc.CSRMatrix.generate(r, c, deg=(d // 2, d + d // 2), l2norm=False, seed=0xACCE1E5 + 10 + i)
+        srv.add_matrix(csr, csr.pack(bits=8, split=32))
+        del csr
+    log(f"[bench] serve setup {time.perf_counter() - t0:.1f}s, {srv.stats()['device_bytes'] / 1e9:.2f} GB resident")
+    runs = {}
+    for rate in (2000, 10000, 50000):
+        st = srv.bench_poisson(rate, a.serve_seconds, kmax=100, seed=rate)
+        runs[str(rate)] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}
+        log(f"[bench] serve {rate} q/s offered: {st['qps']:.0f} q/s done, p50 {st['p50_us']:.0f}us, "
+            f"p99 {st['p99_us']:.0f}us, mean batch {st['mean_batch']:.1f}")
+    srv.close()
+    top = runs["50000"]
+    line = line_common(a, "serving queries/s with p50/p99 latency (continuous batching)", top["qps"], "queries/s",
+                       1e3 / max(top["qps"], 1e-9), "u8",
+                       "serving: 1,000,000x500 deg 60 + 5,000,000x900 deg 40 + 10,000,000x1024 deg 20, 8-bit, CK=512, "
+                       "Poisson arrivals, random matrix, K uniform in [1,100]", {"offered_qps": [2000, 10000, 50000]})
+    line.update({"roofline": None, "cpu_baseline": None, "runs": runs})
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -87,6 +243,10 @@ def main():
               "design is described in DESIGN.md and is not measured here.", file=sys.stderr)
         sys.exit(2)
     import acceles_amd as acc
+    if a.workload != "embedding":
+        stream = acc.Stream(0)
+        {"large-single": bench_large_single, "batched": bench_batched, "serve": bench_serve}[a.workload](a, acc, stream)
+        return
 
     assert acc.device_count() > 0, "no GPU visible"
     stream = acc.Stream(0)

@@ -282,7 +282,8 @@ int acc_gpu_set_batch_mode(acc_gpu_matrix *m, int mode, int min_dense_batch);
  * query per pass, the packet batch and the dense batch (where the matrix
  * allows it) on synthetic queries at batch sizes 1, 2, 4, ... up to max_batch,
  * and keeps the fastest layout per batch size in the handle (builds the dense
- * tiles).  points[i] receives each measured size (up to cap entries; *count
+ * tiles unless the handle was set to ACC_BATCH_SPARSE or _SINGLE, which
+ * keeps the dense layout out).  points[i] receives each measured size (up to cap entries; *count
  * gets the number written); times are per batch in microseconds, -1 where a
  * layout does not apply.  Results do not depend on the layout. */
 typedef struct {
@@ -333,6 +334,7 @@ typedef struct {
     int slots;             /* batches in flight, 1..8 (0 = 4)                       */
     int ck;                /* CK of every query (0 = 512)                           */
     int max_pending;       /* waiting requests before submissions are refused (0 = 65536) */
+    int autotune;          /* 1: acc_gpu_autotune each matrix (K = 100) when it is added */
 } acc_server_opts;
 typedef struct {
     int64_t submitted, completed, refused, batches;
