@@ -32,10 +32,29 @@ __device__ __forceinline__ float dkey_score_f32(uint64_t k) {
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-// Orders LDS traffic between the lanes of one wavefront.
+// Orders LDS traffic between the lanes of one wavefront.  The fence names
+// the LDS address space only: a wavefront's LDS operations complete in order,
+// and a fence over all memory would also wait for every global load in flight
+// (draining the prefetch rings of the streaming kernels).
 __device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront", "local");
     __builtin_amdgcn_wave_barrier();
+}
+
+// Maximum over the wavefront, valid in lane 63 (DPP steps: quad swaps,
+// half-row and row mirrors, then the row 15 / row 31 broadcasts).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int32_t dpp_max_step(int32_t v) {
+    return max(v, __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false));
+}
+__device__ __forceinline__ int32_t wave_max_lane63(int32_t v) {
+    v = dpp_max_step<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+    v = dpp_max_step<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+    v = dpp_max_step<0x141, 0xF>(v);  // row_half_mirror
+    v = dpp_max_step<0x140, 0xF>(v);  // row_mirror
+    v = dpp_max_step<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_max_step<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
 }
 
 __device__ __forceinline__ int mbcnt64(uint64_t mask) {

@@ -79,10 +79,7 @@ struct acc_gpu_matrix {
     int4 *d_items = nullptr;         // per workgroup
     int nitems = 0;
     std::vector<int32_t> part_row_begin;
-    uint64_t *d_thr = nullptr;       // split
-    uint32_t *d_cnt = nullptr;       // split
-    uint64_t *d_cand = nullptr;
-    int64_t *d_cand_off = nullptr;   // split + 1
+    int32_t *d_smax = nullptr;       // per slice: largest INT32 score of the last query
     uint64_t *d_resc = nullptr;      // split * kMaxM
     int32_t *d_crow = nullptr, *d_cscore = nullptr;  // split * kMaxM (candidate outputs)
     int64_t *d_part_slice = nullptr;                 // split + 1
@@ -232,16 +229,15 @@ int launch_lowbit(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *
     a.slices = m->d_slices;
     a.perm = m->d_perm;
     a.items = m->d_items;
+    a.part_slice = m->d_part_slice;
+    a.nvalid = m->d_nvalid;
+    a.smax = m->d_smax;
     a.x = x;
     a.cols = m->cols;
     a.col_bits = m->col_bits;
     a.val_bits = m->bits;
     a.split = m->split;
     a.M = M;
-    a.thr = m->d_thr;
-    a.cnt = m->d_cnt;
-    a.cand = m->d_cand;
-    a.cand_off = m->d_cand_off;
     a.ptr = m->d_ptr;
     a.idx = m->d_idx;
     a.val = m->d_val;
@@ -251,22 +247,23 @@ int launch_lowbit(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *
     a.out = out;
     a.cand_rows_out = crow;
     a.cand_scores_out = cscore;
-    lowbit_fn k = get_lowbit_stage1(m->entry_bits, m->is_signed != 0);
-    if (!k) return fail(ACC_ERR_ARG, "unsupported entry width");
+    lowbit_fn k = get_lowbit_slicemax(m->entry_bits, m->is_signed != 0);
+    lowbit_fn ksel = get_lowbit_select(m->entry_bits, m->is_signed != 0);
+    if (!k || !ksel) return fail(ACC_ERR_ARG, "unsupported entry width");
+    const size_t qlds = (size_t)((m->cols + 15) & ~15);
     const size_t coll = 8 * (size_t)LB_WAVES * 128;  // eight WaveTopM<2> buffers
-    const size_t lds1 = 64 + coll + (size_t)((m->cols + 15) & ~15);
     mark(m, 0, st);
-    hipLaunchKernelGGL(k, dim3(m->nitems), dim3(LB_WG), lds1, st, a);
+    hipLaunchKernelGGL(k, dim3(m->nitems), dim3(LB_WG), 64 + qlds, st, a);
     int rc = check_launch();
     if (rc != ACC_OK) return rc;
     mark(m, 1, st);
-    hipLaunchKernelGGL(get_lowbit_merge(), dim3(m->split), dim3(LB_WG), 64 + coll, st, a);
+    hipLaunchKernelGGL(ksel, dim3(m->split), dim3(LB_WG), 64 + coll + qlds, st, a);
     rc = check_launch();
     mark(m, 2, st);
     if (rc != ACC_OK || K <= 0) return rc;
-    const int R = r_for(K);
+    const int R = pow2_at_least((m->split * M + ACC_WG - 1) / ACC_WG);  // split*M <= kMaxCK = 4*64*16
     lowbit_fn f = get_lowbit_final(R);
-    if (!f) return fail(ACC_ERR_ARG, "k too large");
+    if (!f) return fail(ACC_ERR_ARG, "too many candidates");
     hipLaunchKernelGGL(f, dim3(1), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
     mark(m, 3, st);
     return check_launch();
@@ -443,7 +440,8 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     const size_t lds_s = (size_t)m->nkb * NT * 1024 + (size_t)QB * 4;
     if ((rc = lds_limit((const void *)fs, lds_s)) != ACC_OK) return rc;
     const int64_t groups = (m->nslices + 7) / 8;
-    const int nwx = (int)std::max<int64_t>(1, std::min<int64_t>(groups, std::max(1, m->cus / nqb)));
+    // a multiple of 8: the kernel keeps a slice range's query blocks on one XCD
+    const int nwx = (int)std::max<int64_t>(8, std::min<int64_t>(groups, std::max(1, m->cus / nqb)) / 8 * 8);
     hipLaunchKernelGGL(fs, dim3(nwx * nqb), dim3(512), lds_s, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
     mark(m, 2, st);
@@ -613,7 +611,6 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
     int rc = ACC_OK;
     std::vector<int4> items;
     std::vector<int32_t> part_wgs;
-    std::vector<int64_t> coff;
     std::vector<uint2> sl;
 #define TRY(x)                         \
     do {                               \
@@ -671,8 +668,6 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
             sl[(size_t)s] = make_uint2(pk->slice_off[(size_t)s], pk->slice_ndw[(size_t)s]);
         items = make_items(pk, m->cus, part_wgs);
         m->nitems = (int)items.size();
-        coff.assign((size_t)m->split + 1, 0);
-        for (int p = 0; p < m->split; ++p) coff[(size_t)p + 1] = coff[(size_t)p] + (int64_t)part_wgs[(size_t)p] * kMaxM;
         m->stream_bytes = (int64_t)pk->words.size() * 4 + m->nslices * 8;
         // E*64 dwords of zero padding: the packet kernels load whole groups
         // unconditionally and mask what lies past a group's length
@@ -680,10 +675,7 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         TRY(dalloc(m, &m->d_slices, sl.size()));
         TRY(dalloc(m, &m->d_perm, pk->perm.size()));
         TRY(dalloc(m, &m->d_items, items.size()));
-        TRY(dalloc(m, &m->d_thr, (size_t)m->split));
-        TRY(dalloc(m, &m->d_cnt, (size_t)m->split));
-        TRY(dalloc(m, &m->d_cand, (size_t)coff[(size_t)m->split]));
-        TRY(dalloc(m, &m->d_cand_off, coff.size()));
+        TRY(dalloc(m, &m->d_smax, (size_t)m->nslices));
         TRY(dalloc(m, &m->d_resc, (size_t)m->split * kMaxM));
         TRY(dalloc(m, &m->d_crow, (size_t)m->split * kMaxM));
         TRY(dalloc(m, &m->d_cscore, (size_t)m->split * kMaxM));
@@ -694,9 +686,6 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         COPY(m->d_slices, sl.data(), sl.size() * sizeof(uint2));
         COPY(m->d_perm, pk->perm.data(), pk->perm.size() * 4);
         COPY(m->d_items, items.data(), items.size() * sizeof(int4));
-        COPY(m->d_cand_off, coff.data(), coff.size() * 8);
-        ZERO(m->d_thr, (size_t)m->split * 8);
-        ZERO(m->d_cnt, (size_t)m->split * 4);
         {
             // valid rows per slice: padding lanes (perm -1) close a partition's last slice
             std::vector<uint8_t> nv((size_t)m->nslices, 0);

@@ -37,10 +37,11 @@ __global__ __launch_bounds__(ACC_WG) void batch_scores(BatchArgs a) {
 #pragma unroll
         for (int i = 0; i < QB; ++i) acc[i] = 0u;
         int32_t vsum = 0;
+        const __amdgpu_buffer_rsrc_t rs = slice_rsrc(a.words, sl);
         for (int g0 = 0; g0 < (int)sl.y; g0 += E) {
             uint32_t w[E];
             const int nd = min(E, (int)sl.y - g0);
-            load_group<E>(w, a.words + ((size_t)sl.x + g0) * 64 + lane, nd);
+            load_group<E>(w, rs, g0);
 #pragma unroll
             for (int j = 0; j < 32; ++j) {
                 const int pos = j * E, wi = pos >> 5, sh = pos & 31;

@@ -160,9 +160,9 @@ __global__ __launch_bounds__(ACC_WG) void dense_prep(DenseArgs a) {
     }
 }
 
-// Scores of 64-row slices against a block of QB = 32*NT queries.  Workgroup
-// w serves query block w % nqb (so the blocks of one slice range run side by
-// side and share its tiles in L2) and slices (w / nqb)*8 + wave + k*8*(grid/nqb).
+// Scores of 64-row slices against a block of QB = 32*NT queries.  A
+// workgroup serves one query block and the slices wx*8 + wave + k*8*nwx of
+// its slice range wx (grid = nwx * nqb, nwx a multiple of 8).
 // Per slice a wavefront runs 2*NT MFMA chains over the nkb column blocks, the
 // next P blocks of the matrix tiles in flight; the B operands come from the
 // query table in LDS.
@@ -177,8 +177,12 @@ __global__ __launch_bounds__(512) void dense_scores(DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int QB = 32 * NT;
     const int nkb = a.nkb, nqb = a.nqb;
-    const int qb = blockIdx.x % nqb;
-    const int wx = blockIdx.x / nqb, nwx = gridDim.x / nqb;
+    // workgroups are dealt round-robin to the 8 XCDs: the nqb query blocks of
+    // one slice range (same wx) get ids b, b+8, b+16, ..., hence the same XCD,
+    // and the second and later blocks find the tiles in that XCD's L2
+    const int c = blockIdx.x % (8 * nqb);
+    const int qb = c / 8;
+    const int wx = (blockIdx.x / (8 * nqb)) * 8 + (c % 8), nwx = gridDim.x / nqb;
     v4i *qt = (v4i *)smem;
     int32_t *corr = (int32_t *)(smem + (size_t)nkb * NT * 1024);
     {
@@ -498,61 +502,9 @@ template <int R>
 __global__ __launch_bounds__(ACC_WG) void batch_topk(DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_nz[ACC_WAVES];
-    constexpr int N = 64 * R;  // keys per wavefront list
-    uint64_t *lists = (uint64_t *)smem;
-    const int wave = threadIdx.x >> 6, lane = lane_id();
     const int q = blockIdx.x;
     const int total = a.split * a.M;
-    const uint64_t *keys = a.resc + (size_t)q * total;
-    uint64_t v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int i = wave * N + r * 64 + lane;
-        v[r] = i < total ? keys[i] : 0ull;
-    }
-    reg_sort_desc<R>(v);
-    int nz = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        lists[wave * N + r * 64 + lane] = v[r];
-        nz += v[r] != 0ull;
-    }
-    for (int o = 32; o > 0; o >>= 1) nz += __shfl_xor(nz, o);
-    if (lane == 0) s_nz[wave] = nz;
-    __syncthreads();
-    const int K = a.K[q];
-    acc_element *out = a.out + (size_t)q * a.out_stride;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const uint64_t key = v[r];
-        if (key == 0ull) continue;
-        int rank = r * 64 + lane;  // its place in its own (sorted) list
-#pragma unroll
-        for (int w = 0; w < ACC_WAVES; ++w) {
-            if (w == wave) continue;
-            const uint64_t *L = lists + w * N;
-            int lo = 0, hi = N;  // number of keys of L above key
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (L[mid] > key) lo = mid + 1;
-                else hi = mid;
-            }
-            rank += lo;
-        }
-        if (rank < K) {
-            acc_element e;
-            e.value = dkey_score_f32(key);
-            e.index = dkey_row(key);
-            out[rank] = e;
-        }
-    }
-    const int n = s_nz[0] + s_nz[1] + s_nz[2] + s_nz[3];
-    for (int i = n + (int)threadIdx.x; i < K; i += ACC_WG) {
-        acc_element e;
-        e.value = -__builtin_inff();
-        e.index = -1;
-        out[i] = e;
-    }
+    topk_by_rank<R>(a.resc + (size_t)q * total, total, a.K[q], a.out + (size_t)q * a.out_stride, (uint64_t *)smem, s_nz);
 }
 
 dense_fn get_dense_build() { return dense_build; }

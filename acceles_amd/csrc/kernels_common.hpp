@@ -2,17 +2,19 @@
 // shared by the HIP translation units of the Top-K SpMV path.
 //
 // A low-bit query is three launches on one stream:
-//   lowbit_stage1   query quantization (quantify_util.py:31-49, 259) fused
-//                   with the INT32 SpMV over sliced packets, a per-wavefront
-//                   Top-M, the workgroup merge and the append of what can
-//                   still be in the partition's Top-M (quantify_util.py:173-196;
-//                   spmv_ucsr_top_k_multicore.hpp:195-413 on the FPGA);
-//   lowbit_merge    per partition: its Top-M (host_spmv_ucsr.cpp:691-748) and
-//                   the FP32 rescoring of those rows (quantify_util.py:354-367);
+//   lowbit_slicemax query quantization (quantify_util.py:31-49, 259) and the
+//                   INT32 SpMV over sliced packets (quantify_util.py:5-6;
+//                   spmv_ucsr_top_k_multicore.hpp:195-355 on the FPGA),
+//                   keeping the maximum of each 64-row slice;
+//   lowbit_select   per partition: exact threshold from the slice maxima,
+//                   rescan of the slices that reach it, its Top-M
+//                   (quantify_util.py:173-196, host_spmv_ucsr.cpp:691-748)
+//                   and the FP32 rescoring of those rows
+//                   (quantify_util.py:354-367);
 //   lowbit_final    the global Top-K of the rescored candidates.
-// Cross-workgroup data moves only through atomics and kernel boundaries: a
-// device-scope fence on a multi-XCD part writes back and invalidates the
-// whole L2, which costs microseconds per workgroup.
+// Cross-workgroup data moves only through kernel boundaries: a device-scope
+// fence on a multi-XCD part writes back and invalidates the whole L2, which
+// costs microseconds per workgroup.
 //   lowbit_scores   INT32 scores of every row (parity checks).
 //   fp32_spmv       plain FP32 CSR SpMV (cpu_mkl_baseline.cpp:261).
 //   select_blocks / select_final
@@ -56,11 +58,23 @@ __device__ inline void quantize_query_lds(const float *x, int cols, int8_t *qx, 
 // sliced-packet decoding
 // ---------------------------------------------------------------------------
 
-// Loads one group (E dwords per lane = up to 32 entries); dwords >= nd are 0.
+// Buffer resource over one slice's packets (64 lanes x ndw dwords, lane-
+// interleaved): a load past the slice's end returns 0 without touching
+// memory, so group loads can be unconditional and still fetch exactly the
+// slice's bytes.  off and ndw are wave-uniform.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slice_rsrc(const uint32_t *words, uint2 sl) {
+    const uint32_t off = __builtin_amdgcn_readfirstlane(sl.x), ndw = __builtin_amdgcn_readfirstlane(sl.y);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(words + (size_t)off * 64), (short)0, (int)(ndw * 256u),
+                                             0x00020000 /* 32-bit data format */);
+}
+
+// Loads group g0 / E (E dwords per lane = up to 32 entries) of a slice,
+// non-temporal; dwords past the slice's end are 0.
 template <int E>
-__device__ __forceinline__ void load_group(uint32_t (&w)[E], const uint32_t *lanebase, int nd) {
+__device__ __forceinline__ void load_group(uint32_t (&w)[E], __amdgpu_buffer_rsrc_t rs, int g0) {
+    const int lane = lane_id();
 #pragma unroll
-    for (int i = 0; i < E; ++i) w[i] = (i < nd) ? __builtin_nontemporal_load(lanebase + i * 64) : 0u;
+    for (int i = 0; i < E; ++i) w[i] = __builtin_amdgcn_raw_buffer_load_b32(rs, ((g0 + i) * 64 + lane) * 4, 0, 2 /* nt */);
 }
 
 // Accumulates value * q[col] over the entries of one group.  Padding entries
@@ -81,7 +95,7 @@ __device__ __forceinline__ int32_t decode_group(const uint32_t (&w)[E], int nd, 
             const uint32_t bits = (sh + E <= 32) ? (w[wi] >> sh) : __builtin_amdgcn_alignbit(hi, w[wi], sh);
             const int col = (int)(bits & cmask);
             const int val = SIGNED ? __builtin_amdgcn_sbfe(bits, cb, vb) : (int)__builtin_amdgcn_ubfe(bits, cb, vb);
-            acc += val * (int)qx[col];
+            acc += __mul24(val, (int)qx[col]);  // |val|, |q| < 2^8: v_mul_i32_i24 (a full-rate op; a 32-bit multiply is quarter rate)
         }
     }
     return acc;
@@ -172,6 +186,67 @@ __device__ __forceinline__ float group_dot_rt(const uint32_t *ptr, const int32_t
 }
 
 
+// One workgroup of ACC_WG threads: the K best of total <= ACC_WAVES*64*R
+// ranking keys (0 = empty) into out[0, K), then (value -inf, index -1) past
+// the nonzero keys.  Each wavefront sorts its 64*R keys in registers and
+// stores them in LDS (lists: ACC_WAVES*64*R keys); every key finds its rank
+// in the union by binary search in the other lists (keys are distinct: one
+// per row).
+template <int R>
+__device__ inline void topk_by_rank(const uint64_t *keys, int total, int K, acc_element *out, uint64_t *lists,
+                                    int *s_nz) {
+    constexpr int N = 64 * R;  // keys per wavefront list
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = wave * N + r * 64 + lane;
+        v[r] = i < total ? keys[i] : 0ull;
+    }
+    reg_sort_desc<R>(v);
+    int nz = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        lists[wave * N + r * 64 + lane] = v[r];
+        nz += v[r] != 0ull;
+    }
+    for (int o = 32; o > 0; o >>= 1) nz += __shfl_xor(nz, o);
+    if (lane == 0) s_nz[wave] = nz;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint64_t key = v[r];
+        if (key == 0ull) continue;
+        int rank = r * 64 + lane;  // its place in its own (sorted) list
+#pragma unroll
+        for (int w = 0; w < ACC_WAVES; ++w) {
+            if (w == wave) continue;
+            const uint64_t *L = lists + w * N;
+            int lo = 0, hi = N;  // number of keys of L above key
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (L[mid] > key) lo = mid + 1;
+                else hi = mid;
+            }
+            rank += lo;
+        }
+        if (rank < K) {
+            acc_element e;
+            e.value = dkey_score_f32(key);
+            e.index = dkey_row(key);
+            out[rank] = e;
+        }
+    }
+    int n = 0;
+    for (int w = 0; w < ACC_WAVES; ++w) n += s_nz[w];
+    for (int i = n + (int)threadIdx.x; i < K; i += ACC_WG) {
+        acc_element e;
+        e.value = -__builtin_inff();
+        e.index = -1;
+        out[i] = e;
+    }
+}
+
 template <int R>
 __device__ inline void write_results(const WaveTopM<R> &col, int K, acc_element *out) {
     for (int i = lane_id(); i < K; i += 64) {
@@ -183,20 +258,17 @@ __device__ inline void write_results(const WaveTopM<R> &col, int K, acc_element 
     }
 }
 
-// One workgroup, after lowbit_merge: the global Top-K of the rescored
-
 struct LowbitArgs {
     const uint32_t *words;      // lane-interleaved packet dwords
     const uint2 *slices;        // (offset in 64-word units, dwords per lane)
     const int32_t *perm;        // slot -> row (-1: padding lane)
     const int4 *items;          // per workgroup: (partition, first slice, end slice, 0)
+    const int64_t *part_slice;  // split + 1 slice offsets
+    const uint8_t *nvalid;      // rows per slice (64 except a partition's last)
+    int32_t *smax;              // per slice: largest INT32 score of its lanes
     const float *x;             // FP32 query (cols)
     int cols, col_bits, val_bits;
     int split, M;               // partitions, rows kept per partition
-    uint64_t *thr;              // per partition: lower bound of its M-th key
-    uint32_t *cnt;              // per partition: candidate list length
-    uint64_t *cand;             // candidate lists
-    const int64_t *cand_off;    // per partition offset into cand
     const uint32_t *ptr;        // FP32 CSR for rescoring
     const int32_t *idx;
     const float *val;
@@ -339,8 +411,8 @@ typedef void (*spmv_fn)(const uint32_t *, const int32_t *, const float *, const 
 typedef void (*reset_fn)(uint64_t *, uint32_t *);
 typedef void (*fp64_fn)(const uint32_t *, const int32_t *, const float *, const float *, int, double *);
 
-lowbit_fn get_lowbit_stage1(int E, bool is_signed);  // k_lowbit.hip
-lowbit_fn get_lowbit_merge();
+lowbit_fn get_lowbit_slicemax(int E, bool is_signed);  // k_lowbit.hip
+lowbit_fn get_lowbit_select(int E, bool is_signed);
 lowbit_fn get_lowbit_final(int R);
 scores_fn get_lowbit_scores(int E, bool is_signed);
 spmv_fn get_fp32_spmv(int G);                         // k_fp32.hip

@@ -149,6 +149,7 @@ def bench_large_single(a, acc, stream):
     fqps = 1e3 / float(np.sum(fp_ms))
     for j in range(nq):
         g.topk_lowbit(xs[j], K, CK, os_[j], stream)
+    stream.synchronize()
     res = out.numpy()
     prec = [precision_at(res[j, :, 1], exact_topk(g, acc, xs[j], rows, K, stream)) for j in range(min(nq, 8))]
     precision = dict(zip(map(str, KS), np.round(np.mean(prec, axis=0), 5).tolist()))
