@@ -219,7 +219,9 @@ int check_launch() {
     return ACC_OK;
 }
 
-// One low-bit query (stage 1, per-partition merge + rescoring, final Top-K):
+int lds_limit(const void *fn, size_t bytes);
+
+// One low-bit query (slice maxima, per-partition select + rescoring, final Top-K):
 // Top-K into out (K > 0) and/or the per-partition candidates into
 // crow/cscore (p*M + i).
 int launch_lowbit(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *out, int32_t *crow, int32_t *cscore,
@@ -251,20 +253,26 @@ int launch_lowbit(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *
     lowbit_fn ksel = get_lowbit_select(m->entry_bits, m->is_signed != 0);
     if (!k || !ksel) return fail(ACC_ERR_ARG, "unsupported entry width");
     const size_t qlds = (size_t)((m->cols + 15) & ~15);
-    const size_t coll = 8 * (size_t)LB_WAVES * 128;  // eight WaveTopM<2> buffers
+    const size_t lds_sel = 64 + 8 * 4096 + qlds;              // key list, query
+    int rc = lds_limit((const void *)ksel, lds_sel + 40960);  // plus about 36 KB static
+    if (rc != ACC_OK) return rc;
     mark(m, 0, st);
     hipLaunchKernelGGL(k, dim3(m->nitems), dim3(LB_WG), 64 + qlds, st, a);
-    int rc = check_launch();
+    rc = check_launch();
     if (rc != ACC_OK) return rc;
     mark(m, 1, st);
-    hipLaunchKernelGGL(ksel, dim3(m->split), dim3(LB_WG), 64 + coll + qlds, st, a);
+    hipLaunchKernelGGL(ksel, dim3(m->split), dim3(SEL_WG), lds_sel, st, a);
     rc = check_launch();
     mark(m, 2, st);
     if (rc != ACC_OK || K <= 0) return rc;
-    const int R = pow2_at_least((m->split * M + ACC_WG - 1) / ACC_WG);  // split*M <= kMaxCK = 4*64*16
-    lowbit_fn f = get_lowbit_final(R);
-    if (!f) return fail(ACC_ERR_ARG, "too many candidates");
-    hipLaunchKernelGGL(f, dim3(1), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
+    if (K <= 256 && m->split * M <= 4 * SEL_WG) {
+        hipLaunchKernelGGL(get_lowbit_final_radix(), dim3(1), dim3(SEL_WG), 0, st, a);
+    } else {
+        const int R = pow2_at_least((m->split * M + ACC_WG - 1) / ACC_WG);  // split*M <= kMaxCK = 4*64*16
+        lowbit_fn f = get_lowbit_final(R);
+        if (!f) return fail(ACC_ERR_ARG, "too many candidates");
+        hipLaunchKernelGGL(f, dim3(1), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
+    }
     mark(m, 3, st);
     return check_launch();
 }

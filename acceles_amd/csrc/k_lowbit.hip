@@ -106,110 +106,223 @@ __global__ __launch_bounds__(LB_WG) void lowbit_slicemax(LowbitArgs a) {
     }
 }
 
-// One workgroup per partition p = blockIdx.x, after lowbit_slicemax.
-// 1. ts = the M-th largest maximum among the partition's full slices: each
-//    of those M slices holds a distinct row scoring >= ts, so every row of
-//    the partition's Top-M scores >= ts and lies in a slice whose maximum
-//    is >= ts.
-// 2. The slices whose maximum reaches ts are rescanned exactly and their
-//    rows >= ts offered to the Top-M collectors.  Their indices are listed
-//    in LDS (up to kList; past that, each wavefront rescans its own chunks
-//    one slice at a time), the list is processed kRescan slices per
-//    wavefront at a time so that their loads overlap.
+// One workgroup of SEL_WG threads per partition p = blockIdx.x, after
+// lowbit_slicemax.
+// 1. ts = the M-th largest maximum among the partition's full slices (radix
+//    selection): each of those M slices holds a distinct row scoring >= ts,
+//    so every row of the partition's Top-M scores >= ts and lies in a slice
+//    whose maximum is >= ts.
+// 2. The slices whose maximum reaches ts are listed in LDS and rescanned
+//    exactly, CS at a time: their groups are spread over all wavefronts
+//    (kU loads in flight per wavefront) and the partial scores added into
+//    per-slice LDS sums, so long slices do not serialize on one wavefront.
+//    Rows >= ts are appended to an LDS key list, which is cut back to its
+//    best M whenever another chunk might not fit.
+// 3. The list's best M by radix selection, ranked by counting.  Past kList
+//    candidate slices (heavy ties) each wavefront instead rescans its share
+//    slice by slice into a wavefront Top-M collector.
+// 4. Optional candidate output and the FP32 rescoring of the M rows.
 template <int E, bool SIGNED>
-__global__ __launch_bounds__(LB_WG) void lowbit_select(LowbitArgs a) {
-    constexpr int kList = 1024, kRescan = 4;
+__global__ __launch_bounds__(SEL_WG) void lowbit_select(LowbitArgs a) {
+    constexpr int V = 8;         // slice maxima per thread held in registers
+    constexpr int kList = 2048;  // candidate slices listed in LDS
+    constexpr int CS = 32;       // candidate slices per rescan chunk
+    constexpr int kU = E <= 17 ? 3 : 2;  // groups in flight per wavefront (128 VGPRs at 16 waves)
+    constexpr int KCAP = 4096;   // LDS key list (rows >= ts)
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int s_n[LB_WAVES];
-    __shared__ int s_m, s_cnt;
-    __shared__ int32_t s_ts;
+    __shared__ uint32_t s_hist[512];
     __shared__ int32_t s_list[kList];
     __shared__ uint2 s_lmeta[kList];
+    __shared__ int32_t s_sum[CS * 64];
+    __shared__ int s_ustart[CS + 1];
+    __shared__ int s_n[SEL_WAVES];
+    __shared__ int s_m, s_cnt, s_kn, s_tn, s_need[2];
+    __shared__ uint32_t s_t32[2];
+    __shared__ uint64_t s_t64[2];
+    __shared__ uint64_t s_tmp[64], s_top[64];
     float *red = (float *)smem;
-    uint64_t *colbuf = (uint64_t *)(smem + 64);
-    int8_t *qx = (int8_t *)(smem + 64 + 8 * LB_WAVES * WaveTopM<2>::cap);
-    if (threadIdx.x == 0) s_cnt = 0;
-    quantize_query_lds(a.x, a.cols, qx, red);
-    const int wave = threadIdx.x >> 6, lane = lane_id();
+    uint64_t *keys = (uint64_t *)(smem + 64);  // KCAP keys; the slow path's collectors reuse it
+    int8_t *qx = (int8_t *)(smem + 64 + 8 * KCAP);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int p = blockIdx.x;
     const int64_t s0 = a.part_slice[p];
     const int nsl = (int)(a.part_slice[p + 1] - s0);
     const int32_t *smax = a.smax + s0;
     const uint8_t *nvalid = a.nvalid + s0;
     const uint32_t cmask = (1u << a.col_bits) - 1u;
-
-    // 1. threshold (keys (max, slice) are distinct and nonzero)
-    WaveTopM<2> col;
-    col.init(colbuf + wave * WaveTopM<2>::cap, a.M);
-    {
-        constexpr int U = 8;
-        for (int b = wave * 64; b < nsl; b += LB_WG * U) {
-            int32_t v[U];
-            int nv[U];
+    const bool regs = nsl <= V * SEL_WG;
+    int32_t sm[V];
+    bool full[V];
+    if (regs) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = min(b + u * LB_WG + lane, nsl - 1);
-                v[u] = smax[i];
-                nv[u] = nvalid[i];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = b + u * LB_WG + lane;
-                col.push(i < nsl && nv[u] == 64, ((uint64_t)((uint32_t)v[u] ^ 0x80000000u) << 32) | (uint32_t)(i + 1));
-            }
+        for (int u = 0; u < V; ++u) {
+            const int i = min(tid + u * SEL_WG, nsl - 1);
+            sm[u] = nsl > 0 ? smax[i] : 0;
+            full[u] = nsl > 0 && nvalid[i] == 64;
         }
     }
-    col.compact();
-    if (lane == 0) s_n[wave] = col.n;
-    __syncthreads();
-    if (wave == 0) {
-        wg_merge(col, colbuf, s_n, LB_WAVES);
-        if (lane == 0) s_ts = col.n >= a.M ? (int32_t)((uint32_t)(col.buf[a.M - 1] >> 32) ^ 0x80000000u) : INT32_MIN;
+    if (tid == 0) {
+        s_cnt = 0;
+        s_kn = 0;
+        s_tn = 0;
     }
-    __syncthreads();
-    const int32_t ts = s_ts;
+    quantize_query_lds(a.x, a.cols, qx, red);
+
+    // 1. threshold (unsigned order: score ^ 2^31; key 0 = not a full slice)
+    auto mkey = [](int32_t v, bool ok) -> uint32_t { return ok ? ((uint32_t)v ^ 0x80000000u) : 0u; };
+    const uint32_t T = wg_radix_kth<uint32_t>(
+        [&](auto fn) {
+            if (regs) {
+#pragma unroll
+                for (int u = 0; u < V; ++u) fn(tid + u * SEL_WG < nsl ? mkey(sm[u], full[u]) : 0u);
+            } else {
+                for (int i = tid; i < nsl; i += SEL_WG) fn(mkey(smax[i], nvalid[i] == 64));
+            }
+        },
+        a.M, s_hist, s_t32, s_need);
+    const int32_t ts = T == 0u ? INT32_MIN : (int32_t)(T ^ 0x80000000u);
 
     // 2. candidate slices
-    for (int i = threadIdx.x; i < nsl; i += LB_WG) {
-        if (smax[i] >= ts) {
-            const int j = atomicAdd(&s_cnt, 1);
-            if (j < kList) s_list[j] = i;
+    auto offer = [&](int i, bool cand) {
+        const uint64_t m = __ballot(cand);
+        if (m == 0) return;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&s_cnt, __popcll(m));
+        base = __shfl(base, 0);
+        const int j = base + mbcnt64(m);
+        if (cand && j < kList) s_list[j] = i;
+    };
+    if (regs) {
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int i = tid + u * SEL_WG;
+            offer(i, i < nsl && sm[u] >= ts);
+        }
+    } else {
+        for (int b = 0; b < nsl; b += SEL_WG) {
+            const int i = b + tid;
+            offer(i, i < nsl && smax[min(i, nsl - 1)] >= ts);
         }
     }
     __syncthreads();
     const int ncand = s_cnt;
-    col.init(colbuf + wave * WaveTopM<2>::cap, a.M);
+    int mcount = 0;
     if (ncand <= kList) {
-        for (int j = threadIdx.x; j < ncand; j += LB_WG) s_lmeta[j] = a.slices[s0 + s_list[j]];
-        __syncthreads();
-        for (int j0 = wave * kRescan; j0 < ncand; j0 += LB_WAVES * kRescan) {
-            uint32_t w[kRescan][E];
-            int32_t row[kRescan];
-            uint2 sl[kRescan];
-#pragma unroll
-            for (int u = 0; u < kRescan; ++u) {
-                const bool ok = j0 + u < ncand;
-                const int64_t s = s0 + (ok ? s_list[j0 + u] : 0);
-                sl[u] = ok ? s_lmeta[j0 + u] : make_uint2(0u, 0u);
-                load_group<E>(w[u], slice_rsrc(a.words, sl[u]), 0);
-                row[u] = ok ? a.perm[s * 64 + lane] : -1;
+        // keeps the best M of the key list (all threads; the list is published)
+        auto cut = [&](int n) -> int {
+            const uint64_t Tk = wg_radix_kth<uint64_t>(
+                [&](auto fn) {
+                    for (int i = tid; i < n; i += SEL_WG) fn(keys[i]);
+                },
+                a.M, s_hist, s_t64, s_need);
+            for (int i0 = 0; i0 < n; i0 += SEL_WG) {
+                const int i = i0 + tid;
+                const uint64_t k = i < n ? keys[i] : 0ull;
+                const bool sel = k != 0ull && k >= Tk;
+                const uint64_t m = __ballot(sel);
+                if (m == 0) continue;
+                int base = 0;
+                if (lane == 0) base = atomicAdd(&s_tn, __popcll(m));
+                base = __shfl(base, 0);
+                const int j = base + mbcnt64(m);
+                if (sel && j < 64) s_tmp[j] = k;  // j < M for distinct keys
             }
-#pragma unroll
-            for (int u = 0; u < kRescan; ++u) {
-                int32_t sc = decode_group<E, SIGNED>(w[u], min(E, (int)sl[u].y), qx, a.col_bits, a.val_bits, cmask, 0);
-                if ((int)sl[u].y > E) {  // slices longer than one group
-                    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(a.words, sl[u]);
-                    for (int g0 = E; g0 < (int)sl[u].y; g0 += E) {
-                        uint32_t wg[E];
-                        load_group<E>(wg, rs, g0);
-                        sc = decode_group<E, SIGNED>(wg, min(E, (int)sl[u].y - g0), qx, a.col_bits, a.val_bits, cmask, sc);
-                    }
+            __syncthreads();
+            const int m = min(s_tn, a.M);
+            __syncthreads();
+            if (tid == 0) s_tn = 0;
+            return m;
+        };
+        for (int j = tid; j < ncand; j += SEL_WG) s_lmeta[j] = a.slices[s0 + s_list[j]];
+        for (int cb = 0; cb < ncand; cb += CS) {
+            const int cn = min(CS, ncand - cb);
+            __syncthreads();  // metadata listed; the previous chunk's sums and keys in place
+            if (s_kn > KCAP - CS * 64) {  // the next chunk might not fit: keep the best M
+                const int m = cut(s_kn);
+                if (tid < m) keys[tid] = s_tmp[tid];
+                if (tid == 0) s_kn = m;
+                __syncthreads();
+            }
+            for (int i = tid; i < CS * 64; i += SEL_WG) s_sum[i] = 0;
+            if (wave == 0) {
+                const int ng = lane < cn ? ((int)s_lmeta[cb + lane].y + E - 1) / E : 0;
+                int inc = ng;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(inc, o);
+                    if (lane >= o) inc += t;
                 }
-                col.push(row[u] >= 0 && sc >= ts, dkey_i32(sc, row[u]));
+                if (lane < CS) s_ustart[lane + 1] = inc;
+                if (lane == 0) s_ustart[0] = 0;
+            }
+            constexpr int PR = CS / SEL_WAVES;  // slices each wavefront offers
+            int32_t prow[PR];
+#pragma unroll
+            for (int r = 0; r < PR; ++r) {
+                const int c = wave + r * SEL_WAVES;
+                prow[r] = c < cn ? a.perm[(s0 + s_list[cb + c]) * 64 + lane] : -1;
+            }
+            __syncthreads();
+            const int nu = s_ustart[cn];
+            for (int u0 = wave * kU; u0 < nu; u0 += SEL_WAVES * kU) {
+                uint32_t w[kU][E];
+                int cu[kU], nd[kU];
+#pragma unroll
+                for (int t = 0; t < kU; ++t) {
+                    const int u = u0 + t;
+                    int c = -1, g0 = 0;
+                    uint2 sl = make_uint2(0u, 0u);
+                    if (u < nu) {
+                        int lo = 0, hi = cn - 1;  // last c with s_ustart[c] <= u
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (s_ustart[mid] <= u) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        c = lo;
+                        g0 = (u - s_ustart[c]) * E;
+                        sl = s_lmeta[cb + c];
+                    }
+                    load_group<E>(w[t], slice_rsrc(a.words, sl), g0);
+                    cu[t] = c;
+                    nd[t] = c >= 0 ? min(E, (int)sl.y - g0) : 0;
+                }
+#pragma unroll
+                for (int t = 0; t < kU; ++t) {
+                    if (cu[t] < 0) continue;
+                    const int32_t part = decode_group<E, SIGNED>(w[t], nd[t], qx, a.col_bits, a.val_bits, cmask, 0);
+                    atomicAdd(&s_sum[cu[t] * 64 + lane], part);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < PR; ++r) {
+                const int c = wave + r * SEL_WAVES;
+                if (c < cn) {
+                    const int32_t sc = s_sum[c * 64 + lane];
+                    const bool ok = prow[r] >= 0 && sc >= ts;
+                    const uint64_t m = __ballot(ok);
+                    if (m == 0) continue;
+                    int base = 0;
+                    if (lane == 0) base = atomicAdd(&s_kn, __popcll(m));
+                    base = __shfl(base, 0);
+                    if (ok) keys[base + mbcnt64(m)] = dkey_i32(sc, prow[r]);
+                }
             }
         }
+        __syncthreads();
+        mcount = cut(s_kn);
+        if (tid < mcount) {  // rank by counting (keys are distinct)
+            const uint64_t k = s_tmp[tid];
+            int rank = 0;
+            for (int j = 0; j < mcount; ++j) rank += s_tmp[j] > k;
+            s_top[rank] = k;
+        }
+        __syncthreads();
     } else {
-        for (int b = wave * 64; b < nsl; b += LB_WG) {
+        uint64_t *colbuf = keys;
+        WaveTopM<2> col;
+        col.init(colbuf + wave * WaveTopM<2>::cap, a.M);
+        for (int b = wave * 64; b < nsl; b += SEL_WG) {
             uint64_t mask = __ballot(b + lane < nsl && smax[min(b + lane, nsl - 1)] >= ts);
             while (mask) {
                 const int l = __builtin_ctzll(mask);
@@ -220,42 +333,49 @@ __global__ __launch_bounds__(LB_WG) void lowbit_select(LowbitArgs a) {
                 col.push(row >= 0 && sc >= ts, dkey_i32(sc, row));
             }
         }
+        col.compact();
+        if (lane == 0) s_n[wave] = col.n;
+        __syncthreads();
+        if (wave == 0) {
+            wg_merge(col, colbuf, s_n, SEL_WAVES);
+            if (lane < col.n) s_top[lane] = col.buf[lane];  // col.n <= M <= 64, sorted
+            if (lane == 0) s_m = col.n;
+        }
+        __syncthreads();
+        mcount = s_m;
     }
-    col.compact();
-    if (lane == 0) s_n[wave] = col.n;
-    __syncthreads();
-    if (wave == 0) {
-        wg_merge(col, colbuf, s_n, LB_WAVES);
-        if (lane == 0) s_m = col.n;
-    }
-    __syncthreads();
-    const int mcount = s_m;
-    const uint64_t *top = colbuf;
+    const uint64_t *top = s_top;
     if (a.cand_rows_out) {
-        for (int i = threadIdx.x; i < a.M; i += LB_WG) {
+        for (int i = threadIdx.x; i < a.M; i += SEL_WG) {
             a.cand_rows_out[(size_t)p * a.M + i] = i < mcount ? dkey_row(top[i]) : -1;
             a.cand_scores_out[(size_t)p * a.M + i] = i < mcount ? dkey_score_i32(top[i]) : 0;
         }
     }
     if (a.K <= 0) return;
 
-    // 3. FP32 rescoring of the partition's candidates
-    for (int base = 0; base < a.M; base += LB_WG / a.G) {
+    // 4. FP32 rescoring of the partition's candidates (16 entries per lane in flight)
+    for (int base = 0; base < a.M; base += SEL_WG / a.G) {
         const int i = base + (int)threadIdx.x / a.G;
         const bool valid = i < mcount;
         const int32_t r = valid ? dkey_row(top[i]) : 0;
-        const float sc = group_dot_rt(a.ptr, a.idx, a.val, a.x, r, valid, a.G);
+        const float sc = group_dot_rt<16>(a.ptr, a.idx, a.val, a.x, r, valid, a.G);
         if ((threadIdx.x & (a.G - 1)) == 0 && i < a.M) a.resc[(size_t)p * a.M + i] = valid ? dkey_f32(sc, r) : 0ull;
     }
 }
 
 // One workgroup: the global Top-K of the split*M rescored candidates
-// (split*M <= 4*64*R).
+// (split*M <= 4*64*R), ranks by merging sorted wavefront lists.
 template <int R>
 __global__ __launch_bounds__(ACC_WG) void lowbit_final(LowbitArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_nz[ACC_WAVES];
     topk_by_rank<R>(a.resc, a.split * a.M, a.K, a.out, (uint64_t *)smem, s_nz);
+}
+
+// The same for K <= 256 by radix selection (split*M <= 4*SEL_WG).
+__global__ __launch_bounds__(SEL_WG) void lowbit_final_radix(LowbitArgs a) {
+    __shared__ __align__(16) uint32_t lds[1032];
+    topk_radix<4>(a.resc, a.split * a.M, a.K, a.out, lds);
 }
 
 // INT32 scores of every row (one work item per wavefront).
@@ -321,6 +441,7 @@ lowbit_fn get_lowbit_slicemax(int E, bool is_signed) {
     return is_signed ? slicemax_for<true>(E) : slicemax_for<false>(E);
 }
 lowbit_fn get_lowbit_select(int E, bool is_signed) { return is_signed ? select_for<true>(E) : select_for<false>(E); }
+lowbit_fn get_lowbit_final_radix() { return lowbit_final_radix; }
 lowbit_fn get_lowbit_final(int R) {
     switch (R) {
         case 1: return lowbit_final<1>;

@@ -30,6 +30,8 @@
 #define LB_WAVES 8
 #define LB_WG (LB_WAVES * 64)
 #define LB_MAX_ITEM 1024  // slices per stage-1 work item (their metadata sits in LDS)
+#define SEL_WAVES 16
+#define SEL_WG (SEL_WAVES * 64)
 
 // Quantize the FP32 query into LDS: q = clip(rint(127 * (x / max|x|)), -128, 127)
 // (quantify_vector with B=8, symmetric; float32 division and product, half-even).
@@ -157,27 +159,28 @@ __device__ inline void publish_and_append(WaveTopM<R> &col, uint64_t *thr, uint3
 
 // Dot product of CSR row r with x by a group of G lanes (runtime G): lane g
 // sums entries g, g+G, ... with fmaf, then an xor butterfly of plain adds.
-// Entries are fetched four at a time (then their x values), so a row costs
-// one memory round trip per 4*G entries; the fmaf order is unchanged.
+// Entries are fetched U at a time (then their x values), so a row costs one
+// memory round trip per U*G entries; the fmaf order does not depend on U.
+template <int U = 4>
 __device__ __forceinline__ float group_dot_rt(const uint32_t *ptr, const int32_t *idx, const float *val, const float *x,
                                               int r, bool valid, int G) {
     const int g = lane_id() & (G - 1);
     float p = 0.f;
     if (valid) {
         const uint32_t e = ptr[r + 1];
-        for (uint32_t j = ptr[r] + g; j < e; j += 4 * G) {
-            float v[4], xv[4];
-            int32_t c[4];
+        for (uint32_t j = ptr[r] + g; j < e; j += U * G) {
+            float v[U], xv[U];
+            int32_t c[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 const uint32_t jj = j + u * G;
                 v[u] = jj < e ? val[jj] : 0.f;
                 c[u] = jj < e ? idx[jj] : 0;
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xv[u] = x[c[u]];
+            for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < U; ++u)
                 if (j + u * G < e) p = __builtin_fmaf(v[u], xv[u], p);
         }
     }
@@ -240,6 +243,161 @@ __device__ inline void topk_by_rank(const uint64_t *keys, int total, int K, acc_
     int n = 0;
     for (int w = 0; w < ACC_WAVES; ++w) n += s_nz[w];
     for (int i = n + (int)threadIdx.x; i < K; i += ACC_WG) {
+        acc_element e;
+        e.value = -__builtin_inff();
+        e.index = -1;
+        out[i] = e;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// workgroup radix selection
+// ---------------------------------------------------------------------------
+
+// The need-th largest of the nonzero keys the workgroup's threads offer
+// (each thread calls fn(key) for its keys through for_each(fn); 0 = no key).
+// First the keys' range: the bits above the highest bit in which the
+// smallest and largest key differ are common to all keys.  Then digit by
+// digit (8 bits) below them: an LDS histogram of the keys that share the
+// prefix chosen so far, and wavefront 0 picks the bin that holds the need-th
+// key.  Starting below the common bits spreads the first histogram over the
+// keys' actual range (bins hit by many lanes at once serialize).
+// Returns T with at least need keys >= T; when a bin is taken whole it
+// returns its lower end, so exactly need keys are >= T when the keys are
+// distinct.  Returns 0 when fewer than need keys are offered.
+// hist: 512 LDS words; s_t: 2 LDS keys; s_need: 2 LDS ints.  Called by
+// every thread of the workgroup.
+template <typename T, class F>
+__device__ T wg_radix_kth(F for_each, int need, uint32_t *hist, T *s_t, int *s_need) {
+    constexpr int NB = 8 * (int)sizeof(T);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < 256; i += blockDim.x) hist[i] = 0u;
+    if (tid == 0) {
+        s_t[0] = ~(T)0;
+        s_t[1] = (T)0;
+    }
+    T lo = ~(T)0, hi = 0;
+    for_each([&](T k) {
+        if (k != 0) {
+            lo = k < lo ? k : lo;
+            hi = k > hi ? k : hi;
+        }
+    });
+    for (int o = 32; o > 0; o >>= 1) {
+        T ol, oh;
+        if constexpr (sizeof(T) == 8) {
+            ol = shfl_xor_u64(lo, o);
+            oh = shfl_xor_u64(hi, o);
+        } else {
+            ol = (T)__shfl_xor((int)lo, o);
+            oh = (T)__shfl_xor((int)hi, o);
+        }
+        lo = ol < lo ? ol : lo;
+        hi = oh > hi ? oh : hi;
+    }
+    __syncthreads();
+    if (lane == 0 && hi != 0) {
+        atomicMin(&s_t[0], lo);
+        atomicMax(&s_t[1], hi);
+    }
+    __syncthreads();
+    lo = s_t[0];
+    hi = s_t[1];
+    if (hi == 0) return (T)0;
+    int bits = lo == hi ? 1 : NB - (sizeof(T) == 8 ? __builtin_clzll((uint64_t)(lo ^ hi)) : __builtin_clz((uint32_t)(lo ^ hi)));
+    T prefix = bits >= NB ? (T)0 : (T)(hi >> bits);  // the common high bits
+    int rem = need;
+    for (int pass = 0; bits > 0; ++pass) {
+        const int d = bits < 8 ? bits : 8;
+        const int shift = bits - d;
+        uint32_t *h = hist + (pass & 1) * 256;
+        for_each([&](T k) {
+            if (k != 0 && (shift + d >= NB || (k >> (shift + d)) == prefix))
+                atomicAdd(&h[(uint32_t)(k >> shift) & ((1u << d) - 1u)], 1u);
+        });
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t c[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) c[j] = h[255 - 4 * lane - j];  // lane 0 holds the top bins
+            const uint32_t sum = c[0] + c[1] + c[2] + c[3];
+            uint32_t inc = sum;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
+            }
+            uint32_t above = inc - sum;
+            if (lane == 0) s_need[0] = -1;  // stays when fewer than rem keys share the prefix
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (above < (uint32_t)rem && (uint32_t)rem <= above + c[j]) {
+                    s_t[0] = (T)((prefix << d) | (T)(255 - 4 * lane - j));
+                    s_need[0] = rem - (int)above;
+                    s_need[1] = c[j] == (uint32_t)rem - above;
+                }
+                above += c[j];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) hist[((pass + 1) & 1) * 256 + 4 * lane + j] = 0u;
+        }
+        __syncthreads();
+        rem = s_need[0];
+        if (rem < 0) return (T)0;
+        prefix = s_t[0];
+        bits = shift;
+        if (s_need[1]) return (T)(prefix << shift);
+    }
+    return prefix;
+}
+
+// The K <= 256 best of total <= V * blockDim.x ranking keys (0 = empty)
+// into out[0, K), then (value -inf, index -1) past the nonzero keys: radix
+// selection of the K-th key, then each selected key's rank by counting the
+// selected keys above it.  lds: 4128 bytes (histogram, state, 256 keys).
+template <int V>
+__device__ inline void topk_radix(const uint64_t *keys, int total, int K, acc_element *out, uint32_t *lds) {
+    uint32_t *hist = lds;
+    int *s_need = (int *)(lds + 512);
+    int *s_n = (int *)(lds + 514);
+    uint64_t *s_t = (uint64_t *)(lds + 516);
+    uint64_t *s_sel = (uint64_t *)(lds + 520);
+    const int tid = threadIdx.x, lane = tid & 63;
+    uint64_t k[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+        const int i = tid + u * (int)blockDim.x;
+        k[u] = i < total ? keys[i] : 0ull;
+    }
+    if (tid == 0) *s_n = 0;
+    const uint64_t T = wg_radix_kth<uint64_t>(
+        [&](auto fn) {
+#pragma unroll
+            for (int u = 0; u < V; ++u) fn(k[u]);
+        },
+        K, hist, s_t, s_need);
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+        const bool sel = k[u] != 0ull && k[u] >= T;
+        const uint64_t m = __ballot(sel);
+        if (m == 0) continue;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(s_n, __popcll(m));
+        base = __shfl(base, 0);
+        const int j = base + mbcnt64(m);
+        if (sel && j < 256) s_sel[j] = k[u];  // j < K for distinct keys
+    }
+    __syncthreads();
+    const int n = min(*s_n, 256);  // min(K, nonzero keys)
+    if (tid < n) {
+        const uint64_t key = s_sel[tid];
+        int rank = 0;
+        for (int j = 0; j < n; ++j) rank += s_sel[j] > key;
+        acc_element e;
+        e.value = dkey_score_f32(key);
+        e.index = dkey_row(key);
+        out[rank] = e;
+    }
+    for (int i = n + tid; i < K; i += blockDim.x) {
         acc_element e;
         e.value = -__builtin_inff();
         e.index = -1;
@@ -414,6 +572,7 @@ typedef void (*fp64_fn)(const uint32_t *, const int32_t *, const float *, const 
 lowbit_fn get_lowbit_slicemax(int E, bool is_signed);  // k_lowbit.hip
 lowbit_fn get_lowbit_select(int E, bool is_signed);
 lowbit_fn get_lowbit_final(int R);
+lowbit_fn get_lowbit_final_radix();
 scores_fn get_lowbit_scores(int E, bool is_signed);
 spmv_fn get_fp32_spmv(int G);                         // k_fp32.hip
 select_fn get_select_blocks(int R);

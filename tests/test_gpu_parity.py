@@ -115,6 +115,8 @@ CASES = [
     (5000, 7, (0, 7), 6, 5, True, 0),           # tiny columns, empty rows, 6-bit
     (1000, 4096, (0, 300), 8, 3, False, 0),     # wide rows
     (64, 100, (1, 10), 8, 64, False, 0),        # one row per partition
+    (600000, 64, (2, 8), 8, 1, False, 0),       # one partition of 9375 slices (maxima re-read per digit)
+    (200000, 4, (1, 4), 2, 1, False, 0),        # 2-bit, 4 columns: thousands of slices tie at the top
 ]
 
 
@@ -163,6 +165,22 @@ def test_lowbit_two_stage_bitexact(acc, gpu, rows, cols, deg, bits, split, signe
         _, oi, ov = O.two_stage(ptr, idx, val, q, x, cols, g.info.group, split, ck, k)
         np.testing.assert_array_equal(i, oi)
         np.testing.assert_array_equal(v, ov)
+
+
+@pytest.mark.parametrize("k", [200, 257, 960])
+def test_lowbit_two_stage_large_k(acc, gpu, k):
+    # K above 256 ranks by merging sorted lists, K <= 256 by radix selection
+    rows, cols, deg, bits, split, signed, kind = CASES[0]
+    csr, pk, g = make(acc, rows, cols, deg, bits, split, signed, kind, seed=rows + bits)
+    ptr, idx, val, q = oracle_qv(csr, bits, signed)
+    x = acc.generate_queries(1, cols, seed=k)[0]
+    out = empty(acc, 2 * k, np.int32)
+    g.topk_lowbit(dev(acc, x), k, 1024, out)
+    acc.synchronize()
+    v, i = els(out)
+    _, oi, ov = O.two_stage(ptr, idx, val, q, x, cols, g.info.group, split, 1024, k)
+    np.testing.assert_array_equal(i, oi)
+    np.testing.assert_array_equal(v, ov)
 
 
 def test_lowbit_stream_and_host_form_agree(acc, gpu):
