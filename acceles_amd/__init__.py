@@ -128,6 +128,7 @@ SIGNATURES = [
     ("acc_dev_free", C.c_int, [_P]),
     ("acc_memcpy_h2d", C.c_int, [_P, _P, C.c_size_t]),
     ("acc_memcpy_d2h", C.c_int, [_P, _P, C.c_size_t]),
+    ("acc_memcpy_d2d", C.c_int, [_P, _P, C.c_size_t]),
     ("acc_stream_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("acc_stream_destroy", C.c_int, [_P]),
     ("acc_stream_synchronize", C.c_int, [_P]),
@@ -153,6 +154,7 @@ SIGNATURES = [
     ("acc_gpu_plan_launch", C.c_int, [_P, _P]),
     ("acc_gpu_plan_destroy", None, [_P]),
     ("acc_gpu_stage_times", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
+    ("acc_gpu_time_lowbit_stage", C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_float)]),
     ("acc_gpu_topk_fp32", C.c_int, [_P, _P, C.c_int, _P, _P]),
     ("acc_gpu_topk_lowbit", C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P]),
     ("acc_gpu_spmv_fp64", C.c_int, [_P, _P, _P, _P]),
@@ -440,6 +442,12 @@ class DeviceBuffer:
         _check(lib().acc_memcpy_d2h(out.ctypes.data_as(_P), self.ptr, self.nbytes))
         return out
 
+    def copy_from(self, other):
+        """Device-to-device copy of another buffer of the same size."""
+        if other.nbytes != self.nbytes:
+            raise AccelesError(ACC_ERR_ARG, "copy_from: sizes differ")
+        _check(lib().acc_memcpy_d2d(self.ptr, other.ptr, self.nbytes))
+
     def at(self, offset_elems):
         """Raw device address offset_elems elements into the buffer."""
         return self.ptr.value + int(offset_elems) * self.dtype.itemsize
@@ -625,6 +633,12 @@ class GpuMatrix:
         n = C.c_int()
         _check(lib().acc_gpu_stage_times(self._h, ms, 8, C.byref(n)))
         return [ms[i] for i in range(n.value)]
+
+    def time_lowbit_stage(self, x, k, ck, stage, reps=20, stream=None):
+        """Average ms of one low-bit stage (0 slice maxima, 1 select, 2 final) launched reps times back to back."""
+        ms = C.c_float()
+        _check(lib().acc_gpu_time_lowbit_stage(self._h, _ptr(x), k, ck, stage, reps, _stream(stream), C.byref(ms)))
+        return ms.value
 
     # host-pointer forms (numpy in, numpy out)
     def topk_fp32_host(self, x, k):

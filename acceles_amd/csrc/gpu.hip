@@ -3,6 +3,7 @@
 #include "internal.hpp"
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <chrono>
@@ -101,8 +102,8 @@ struct acc_gpu_matrix {
     int64_t dense_bytes = 0;
     // profiling events around the launches of the last query
     bool profiling = false;
-    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    int nev = 0;  // events recorded by the last query
+    hipEvent_t ev[12] = {};  // start and end of each kernel of the last query
+    int nev = 0;             // events recorded by the last query
     // host-call staging
     float *d_xstage = nullptr;
     acc_element *d_ostage = nullptr;
@@ -160,13 +161,19 @@ int ws_grow(BatchWs &ws, T **p, size_t *cap, size_t need) {
     return grow_in(*ws.allocs, *ws.bytes, p, cap, need);
 }
 
-// Records profiling event i on the query's stream (when profiling is on).
-inline void mark(acc_gpu_matrix *m, int i, hipStream_t st) {
-    if (m->profiling) {
-        hipEventRecord(m->ev[i], st);
-        m->nev = i + 1;
-    }
-}
+// Launches kernel k as stage i of a query.  With profiling on, a pair of
+// HIP events is attached to the dispatch itself (the kernel's start and end),
+// so a stage's time is the kernel's own duration, without launch gaps.
+#define LAUNCH(mm, i, k, grid, block, lds, st, ...)                                                              \
+    do {                                                                                                       \
+        acc_gpu_matrix *m_ = (mm);                                                                             \
+        if (m_ && m_->profiling) {                                                                             \
+            hipExtLaunchKernelGGL(k, grid, block, lds, st, m_->ev[2 * (i)], m_->ev[2 * (i) + 1], 0, __VA_ARGS__); \
+            m_->nev = 2 * (i) + 2;                                                                             \
+        } else {                                                                                               \
+            hipLaunchKernelGGL(k, grid, block, lds, st, __VA_ARGS__);                                          \
+        }                                                                                                      \
+    } while (0)
 
 int pow2_at_least(int v) {
     int p = 1;
@@ -224,8 +231,8 @@ int lds_limit(const void *fn, size_t bytes);
 // One low-bit query (slice maxima, per-partition select + rescoring, final Top-K):
 // Top-K into out (K > 0) and/or the per-partition candidates into
 // crow/cscore (p*M + i).
-int launch_lowbit(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *out, int32_t *crow, int32_t *cscore,
-                  hipStream_t st) {
+LowbitArgs lowbit_args(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *out, int32_t *crow,
+                       int32_t *cscore) {
     LowbitArgs a;
     a.words = m->d_words;
     a.slices = m->d_slices;
@@ -249,32 +256,42 @@ int launch_lowbit(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *
     a.out = out;
     a.cand_rows_out = crow;
     a.cand_scores_out = cscore;
-    lowbit_fn k = get_lowbit_slicemax(m->entry_bits, m->is_signed != 0);
-    lowbit_fn ksel = get_lowbit_select(m->entry_bits, m->is_signed != 0);
-    if (!k || !ksel) return fail(ACC_ERR_ARG, "unsupported entry width");
+    return a;
+}
+
+// Launches stage i of a low-bit query: 0 slice maxima, 1 per-partition
+// select + rescoring, 2 final Top-K (a.K > 0).
+int launch_lowbit_stage(acc_gpu_matrix *m, const LowbitArgs &a, int stage, hipStream_t st) {
     const size_t qlds = (size_t)((m->cols + 15) & ~15);
-    const size_t lds_sel = 64 + 8 * 4096 + qlds;              // key list, query
-    int rc = lds_limit((const void *)ksel, lds_sel + 40960);  // plus about 36 KB static
-    if (rc != ACC_OK) return rc;
-    mark(m, 0, st);
-    hipLaunchKernelGGL(k, dim3(m->nitems), dim3(LB_WG), 64 + qlds, st, a);
-    rc = check_launch();
-    if (rc != ACC_OK) return rc;
-    mark(m, 1, st);
-    hipLaunchKernelGGL(ksel, dim3(m->split), dim3(SEL_WG), lds_sel, st, a);
-    rc = check_launch();
-    mark(m, 2, st);
-    if (rc != ACC_OK || K <= 0) return rc;
-    if (K <= 256 && m->split * M <= 4 * SEL_WG) {
-        hipLaunchKernelGGL(get_lowbit_final_radix(), dim3(1), dim3(SEL_WG), 0, st, a);
+    if (stage == 0) {
+        lowbit_fn k = get_lowbit_slicemax(m->entry_bits, m->is_signed != 0);
+        if (!k) return fail(ACC_ERR_ARG, "unsupported entry width");
+        LAUNCH(m, 0, k, dim3(m->nitems), dim3(LB_WG), 64 + qlds, st, a);
+    } else if (stage == 1) {
+        lowbit_fn ksel = get_lowbit_select(m->entry_bits, m->is_signed != 0);
+        if (!ksel) return fail(ACC_ERR_ARG, "unsupported entry width");
+        const size_t lds_sel = 64 + 8 * 4096 + qlds;              // key list, query
+        int rc = lds_limit((const void *)ksel, lds_sel + 40960);  // plus about 36 KB static
+        if (rc != ACC_OK) return rc;
+        LAUNCH(m, 1, ksel, dim3(m->split), dim3(SEL_WG), lds_sel, st, a);
+    } else if (a.K <= 256 && m->split * a.M <= 4 * SEL_WG) {
+        LAUNCH(m, 2, get_lowbit_final_radix(), dim3(1), dim3(SEL_WG), 0, st, a);
     } else {
-        const int R = pow2_at_least((m->split * M + ACC_WG - 1) / ACC_WG);  // split*M <= kMaxCK = 4*64*16
+        const int R = pow2_at_least((m->split * a.M + ACC_WG - 1) / ACC_WG);  // split*M <= kMaxCK = 4*64*16
         lowbit_fn f = get_lowbit_final(R);
         if (!f) return fail(ACC_ERR_ARG, "too many candidates");
-        hipLaunchKernelGGL(f, dim3(1), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
+        LAUNCH(m, 2, f, dim3(1), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
     }
-    mark(m, 3, st);
     return check_launch();
+}
+
+int launch_lowbit(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *out, int32_t *crow, int32_t *cscore,
+                  hipStream_t st) {
+    const LowbitArgs a = lowbit_args(m, x, M, K, out, crow, cscore);
+    int rc = launch_lowbit_stage(m, a, 0, st);
+    if (rc == ACC_OK) rc = launch_lowbit_stage(m, a, 1, st);
+    if (rc == ACC_OK && K > 0) rc = launch_lowbit_stage(m, a, 2, st);
+    return rc;
 }
 
 // Batched low-bit queries on the packets: batch_scores over query blocks,
@@ -314,20 +331,16 @@ int launch_batch(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
                            : get_batch_scores32(m->entry_bits, m->is_signed != 0, m->bits);
     if (!fs) return fail(ACC_ERR_ARG, "unsupported entry width");
     const size_t lds_q = (size_t)m->cols * (QB + 8);
-    mark(m, 0, st);
-    hipLaunchKernelGGL(fs, dim3(m->nitems, (nq + QB - 1) / QB), dim3(ACC_WG), lds_q, st, a);
+    LAUNCH(m, 0, fs, dim3(m->nitems, (nq + QB - 1) / QB), dim3(ACC_WG), lds_q, st, a);
     int rc = check_launch();
     if (rc != ACC_OK) return rc;
-    mark(m, 1, st);
-    hipLaunchKernelGGL(get_batch_select(), dim3(m->split, nq), dim3(ACC_WG), (size_t)ACC_WAVES * 128 * 8, st, a);
+    LAUNCH(m, 1, get_batch_select(), dim3(m->split, nq), dim3(ACC_WG), (size_t)ACC_WAVES * 128 * 8, st, a);
     rc = check_launch();
-    mark(m, 2, st);
     if (rc != ACC_OK || !out) return rc;
     const int R = r_for(kmax);
     batch_fn ff = get_batch_final(R);
     if (!ff) return fail(ACC_ERR_ARG, "k too large");
-    hipLaunchKernelGGL(ff, dim3(nq), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
-    mark(m, 3, st);
+    LAUNCH(m, 2, ff, dim3(nq), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
     return check_launch();
 }
 
@@ -440,36 +453,30 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     a.cand_rows_out = crow;
     a.cand_scores_out = cscore;
 
-    mark(m, 0, st);
-    hipLaunchKernelGGL(get_dense_prep(), dim3(nqb * QB), dim3(ACC_WG), 0, st, a);
+    LAUNCH(m, 0, get_dense_prep(), dim3(nqb * QB), dim3(ACC_WG), 0, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
-    mark(m, 1, st);
     dense_fn fs = get_dense_scores(NT, m->dense_bytes > kMallBytes);
     const size_t lds_s = (size_t)m->nkb * NT * 1024 + (size_t)QB * 4;
     if ((rc = lds_limit((const void *)fs, lds_s)) != ACC_OK) return rc;
     const int64_t groups = (m->nslices + 7) / 8;
     // a multiple of 8: the kernel keeps a slice range's query blocks on one XCD
     const int nwx = (int)std::max<int64_t>(8, std::min<int64_t>(groups, std::max(1, m->cus / nqb)) / 8 * 8);
-    hipLaunchKernelGGL(fs, dim3(nwx * nqb), dim3(512), lds_s, st, a);
+    LAUNCH(m, 1, fs, dim3(nwx * nqb), dim3(512), lds_s, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
-    mark(m, 2, st);
     const size_t lds_sel = (size_t)ACC_WAVES * 256 * 8 + (size_t)ACC_WAVES * m->cols_pad;
     if ((rc = lds_limit((const void *)get_dense_select(), lds_sel)) != ACC_OK) return rc;
-    hipLaunchKernelGGL(get_dense_select(), dim3(m->split, (nq + ACC_WAVES - 1) / ACC_WAVES), dim3(ACC_WG), lds_sel, st, a);
+    LAUNCH(m, 2, get_dense_select(), dim3(m->split, (nq + ACC_WAVES - 1) / ACC_WAVES), dim3(ACC_WG), lds_sel, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
-    mark(m, 3, st);
     if (!out) return ACC_OK;
     const int64_t threads = (int64_t)cand * m->group;
-    hipLaunchKernelGGL(get_batch_rescore(), dim3((unsigned)((threads + ACC_WG - 1) / ACC_WG)), dim3(ACC_WG), 0, st, a);
+    LAUNCH(m, 3, get_batch_rescore(), dim3((unsigned)((threads + ACC_WG - 1) / ACC_WG)), dim3(ACC_WG), 0, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
-    mark(m, 4, st);
     (void)kmax;
     const int total = m->split * M;  // <= kMaxCK = 4096 = 4 lists of 64*16
     const int R = pow2_at_least((total + 64 * ACC_WAVES - 1) / (64 * ACC_WAVES));
     dense_fn ft = get_batch_topk(R);
     if (!ft) return fail(ACC_ERR_ARG, "too many candidates");
-    hipLaunchKernelGGL(ft, dim3(nq), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
-    mark(m, 5, st);
+    LAUNCH(m, 4, ft, dim3(nq), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
     return check_launch();
 }
 
@@ -502,12 +509,10 @@ int launch_select(const float *y, int n, int K, int blocks, int chunk, uint64_t 
     select_fn fb = get_select_blocks(R), ff = get_select_final(R);
     if (!fb || !ff) return fail(ACC_ERR_ARG, "k too large");
     const size_t lds = (size_t)ACC_WAVES * 64 * R * 8;
-    hipLaunchKernelGGL(fb, dim3(blocks), dim3(ACC_WG), lds, st, s);
+    LAUNCH(prof, 1, fb, dim3(blocks), dim3(ACC_WG), lds, st, s);
     int rc = check_launch();
     if (rc != ACC_OK) return rc;
-    if (prof) mark(prof, 2, st);
-    hipLaunchKernelGGL(ff, dim3(1), dim3(ACC_WG), lds, st, s);
-    if (prof) mark(prof, 3, st);
+    LAUNCH(prof, 2, ff, dim3(1), dim3(ACC_WG), lds, st, s);
     return check_launch();
 }
 
@@ -550,6 +555,11 @@ extern "C" int acc_memcpy_h2d(void *dst, const void *src, size_t bytes) {
 
 extern "C" int acc_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return ACC_OK;
+}
+
+extern "C" int acc_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return ACC_OK;
 }
 
@@ -901,8 +911,8 @@ static int launch_spmv_fp32(acc_gpu_matrix *m, const float *x_dev, float *y, hip
     const int rpi = 64 / m->group;
     const int nblocks = (m->rows + rpi - 1) / rpi;
     const int blocks = std::max(1, std::min((nblocks + ACC_WAVES * 4 - 1) / (ACC_WAVES * 4), m->cus * 8));
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(ACC_WG), (size_t)m->cols * 4, st, m->d_ptr, m->d_idx, m->d_val, x_dev,
-                       m->rows, m->cols, y);
+    LAUNCH(m, 0, k, dim3(blocks), dim3(ACC_WG), (size_t)m->cols * 4, st, m->d_ptr, m->d_idx, m->d_val, x_dev, m->rows,
+           m->cols, y);
     return check_launch();
 }
 
@@ -917,10 +927,8 @@ extern "C" int acc_gpu_topk_fp32(acc_gpu_matrix *m, const float *x_dev, int k, a
     if (!x_dev || !out_dev || k < 1 || k > kMaxK) return fail(ACC_ERR_ARG, "k must be in [1, 960]");
     HIP_TRY(hipSetDevice(m->device));
     hipStream_t st = (hipStream_t)stream;
-    mark(m, 0, st);
     int rc = launch_spmv_fp32(m, x_dev, m->d_y, st);
     if (rc != ACC_OK) return rc;
-    mark(m, 1, st);
     return launch_select(m->d_y, m->rows, k, m->sel_blocks, m->sel_chunk, m->d_sthr, m->d_scnt, m->d_scand,
                          out_dev, st, m);
 }
@@ -942,9 +950,36 @@ extern "C" int acc_gpu_stage_times(acc_gpu_matrix *m, float *ms, int n, int *cou
     HIP_TRY(hipSetDevice(m->device));
     HIP_TRY(hipEventSynchronize(m->ev[m->nev - 1]));
     int c = 0;
-    for (int i = 0; i + 1 < m->nev && c < n; ++i, ++c) HIP_TRY(hipEventElapsedTime(&ms[c], m->ev[i], m->ev[i + 1]));
+    for (int i = 0; 2 * i + 1 < m->nev && c < n; ++i, ++c) HIP_TRY(hipEventElapsedTime(&ms[c], m->ev[2 * i], m->ev[2 * i + 1]));
     if (count) *count = c;
     return ACC_OK;
+}
+
+extern "C" int acc_gpu_time_lowbit_stage(acc_gpu_matrix *m, const float *x_dev, int k, int ck, int stage, int reps,
+                                         void *stream, float *avg_ms) {
+    int rc = check_lowbit(m, ck, k);
+    if (rc != ACC_OK) return rc;
+    if (!x_dev || k < 1 || stage < 0 || stage > 2 || reps < 1 || !avg_ms) return fail(ACC_ERR_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(m->device));
+    hipStream_t st = (hipStream_t)stream;
+    const bool prof = m->profiling;
+    m->profiling = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const LowbitArgs a = lowbit_args(m, x_dev, ck / m->split, k, m->d_ostage, nullptr, nullptr);
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(ACC_ERR_DEVICE, "hipEventCreate");
+    // one whole query leaves the state the later stages read (slice maxima, rescored candidates)
+    for (int s = 0; s < 3 && rc == ACC_OK; ++s) rc = launch_lowbit_stage(m, a, s, st);
+    if (rc == ACC_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(ACC_ERR_DEVICE, "hipEventRecord");
+    for (int r = 0; r < reps && rc == ACC_OK; ++r) rc = launch_lowbit_stage(m, a, stage, st);
+    if (rc == ACC_OK && hipEventRecord(e1, st) != hipSuccess) rc = fail(ACC_ERR_DEVICE, "hipEventRecord");
+    float ms = 0.f;
+    if (rc == ACC_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = fail(ACC_ERR_DEVICE, "event timing failed");
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    m->profiling = prof;
+    if (rc == ACC_OK) *avg_ms = ms / reps;
+    return rc;
 }
 
 extern "C" int acc_gpu_top_k_heap(const float *y_dev, int n, int k, acc_element *out_dev, void *stream) {

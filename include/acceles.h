@@ -192,6 +192,7 @@ int acc_dev_malloc(int device, size_t bytes, void **ptr);
 int acc_dev_free(void *ptr);
 int acc_memcpy_h2d(void *dst_dev, const void *src_host, size_t bytes);
 int acc_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes);
+int acc_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes);
 int acc_stream_create(int device, void **stream);
 int acc_stream_destroy(void *stream);
 int acc_stream_synchronize(void *stream);
@@ -223,13 +224,25 @@ int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info);
 
 /* Per-launch timing with profiling events (CL_QUEUE_PROFILING_ENABLE and
  * get_event_execution_time, opencl_utils.hpp:294,335-344; the per-phase
- * clocks of host_spmv_ucsr.cpp:569-572).  While enabled, every query records
- * HIP events around each of its launches on the query's stream;
- * acc_gpu_stage_times waits for the last query and returns up to n launch
- * durations in ms (low-bit: stage 1, merge+rescore, final Top-K; FP32:
- * SpMV, select, final select) and their count. */
+ * clocks of host_spmv_ucsr.cpp:569-572).  While enabled, every kernel of a
+ * query is launched with a pair of HIP events attached to its dispatch (the
+ * kernel's own start and end); acc_gpu_stage_times waits for the last query
+ * and returns up to n kernel durations in ms (low-bit: slice maxima, select
+ * + rescoring, final Top-K; batched packets: scores, select, final; batched
+ * dense tiles: prep, scores, select, rescoring, final; FP32: SpMV, select,
+ * final select) and their count. */
 int acc_gpu_set_profiling(acc_gpu_matrix *m, int enable);
 int acc_gpu_stage_times(acc_gpu_matrix *m, float *ms, int n, int *count);
+
+/* Average duration of one stage of a low-bit query (0 slice maxima, 1
+ * per-partition select + rescoring, 2 final Top-K) launched reps times back
+ * to back between two HIP events on the stream, after one whole query that
+ * leaves the state the later stages read.  Per-dispatch events (above) add
+ * the dispatch overhead to each kernel; this measures a kernel as it runs
+ * inside back-to-back queries.  Waits for the stream.  No counterpart in the
+ * reference (its phases are timed by host_spmv_ucsr.cpp:569-572 clocks). */
+int acc_gpu_time_lowbit_stage(acc_gpu_matrix *m, const float *x_dev, int k, int ck, int stage, int reps, void *stream,
+                              float *avg_ms);
 void acc_gpu_matrix_destroy(acc_gpu_matrix *m);
 
 /* Plain FP32 CSR Top-K: SpMV (cpu_mkl_baseline.cpp:261) fused with the Top-k
