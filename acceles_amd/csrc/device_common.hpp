@@ -47,6 +47,20 @@ template <int CTRL, int ROWS>
 __device__ __forceinline__ int32_t dpp_max_step(int32_t v) {
     return max(v, __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false));
 }
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_umax_step(uint32_t v) {
+    return max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWS, 0xF, false));
+}
+// Unsigned maximum over the wavefront, returned to every lane.
+__device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
+    v = dpp_umax_step<0xB1, 0xF>(v);
+    v = dpp_umax_step<0x4E, 0xF>(v);
+    v = dpp_umax_step<0x141, 0xF>(v);
+    v = dpp_umax_step<0x140, 0xF>(v);
+    v = dpp_umax_step<0x142, 0xA>(v);
+    v = dpp_umax_step<0x143, 0xC>(v);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
 __device__ __forceinline__ int32_t wave_max_lane63(int32_t v) {
     v = dpp_max_step<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
     v = dpp_max_step<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
@@ -134,6 +148,15 @@ struct WaveTopM {
     // Sort, keep the best M, set the admission threshold.
     __device__ void compact() {
         const int lane = lane_id();
+        if (R > 1 && n <= 64) {  // one register of keys: a 64-key sort
+            uint64_t w[1] = {lane < n ? buf[lane] : 0ull};
+            reg_sort_desc<1>(w);
+            n = n < M ? n : M;
+            if (lane < n) buf[lane] = w[0];
+            if (n == M && M > 0) thr = shfl_u64(w[0], M - 1);
+            wave_sync();
+            return;
+        }
         uint64_t v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = (r * 64 + lane < n) ? buf[r * 64 + lane] : 0ull;

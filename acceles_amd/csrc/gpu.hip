@@ -28,7 +28,7 @@ static constexpr int kMaxK = 960;    // Top-K limit of the FP32 path (collector 
 static constexpr int kMaxCK = 4096;  // candidates of the low-bit path
 static constexpr size_t kWordPad = 32 * 64;  // dwords past the packets (>= widest group, 22 x 64)
 static constexpr size_t kDenseLds = 160 * 1024 - 1024;  // LDS of a dense_scores workgroup
-static constexpr int64_t kMallBytes = 192ll << 20;      // dense tiles that stay cache resident (256 MB MALL)
+static constexpr int64_t kMallBytes = 240ll << 20;      // dense tiles read with normal loads (256 MB MALL)
 
 // Per-batch device workspace of the batched path.  The handle owns one;
 // every plan (captured graph) owns its own, so replaying a plan never touches
@@ -471,12 +471,11 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     const int64_t threads = (int64_t)cand * m->group;
     LAUNCH(m, 3, get_batch_rescore(), dim3((unsigned)((threads + ACC_WG - 1) / ACC_WG)), dim3(ACC_WG), 0, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
-    (void)kmax;
     const int total = m->split * M;  // <= kMaxCK = 4096 = 4 lists of 64*16
     const int R = pow2_at_least((total + 64 * ACC_WAVES - 1) / (64 * ACC_WAVES));
-    dense_fn ft = get_batch_topk(R);
+    dense_fn ft = kmax <= 256 ? get_batch_topk_radix(R) : get_batch_topk(R);
     if (!ft) return fail(ACC_ERR_ARG, "too many candidates");
-    LAUNCH(m, 4, ft, dim3(nq), dim3(ACC_WG), (size_t)ACC_WAVES * 64 * R * 8, st, a);
+    LAUNCH(m, 4, ft, dim3(nq), dim3(ACC_WG), kmax <= 256 ? 0 : (size_t)ACC_WAVES * 64 * R * 8, st, a);
     return check_launch();
 }
 

@@ -337,6 +337,37 @@ __device__ __forceinline__ void reg_sort_desc_u32(uint32_t (&v)[R]) {
     }
 }
 
+// The m-th largest (m >= 1, with repeats) of the U values per lane of the
+// wavefront (0 = no value; returns 0 when fewer than m are nonzero), by
+// sorting each lane's values in registers and then extracting the wavefront
+// maximum m times: about 15 instructions per extracted value, against the 36
+// shuffle stages of a 256-key bitonic sort.
+template <int U>
+__device__ __forceinline__ uint32_t wave_kth_u32(uint32_t (&v)[U], int m) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+#pragma unroll
+        for (int j = 0; j + 1 < U - i; ++j) {
+            const uint32_t x = v[j], y = v[j + 1];
+            v[j] = max(x, y);
+            v[j + 1] = min(x, y);
+        }
+    uint32_t t = 0u;
+    for (int it = 0; it < m; ++it) {
+        const uint32_t mx = wave_umax(v[0]);
+        if (mx == 0u) return 0u;
+        const int l = __builtin_ctzll(__ballot(v[0] == mx));
+        if (lane == l) {
+#pragma unroll
+            for (int j = 0; j + 1 < U; ++j) v[j] = v[j + 1];
+            v[U - 1] = 0u;
+        }
+        t = mx;
+    }
+    return t;
+}
+
 // Offers the listed keys of 64*U slices (k4[u] = the list of slice
 // s0 + b + u*64 + lane) scoring >= ts to the collector; a slice whose list
 // may have dropped such a row is rescanned exactly (from the dense tiles).
@@ -429,12 +460,16 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
             uint32_t v[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) v[u] = nv[u] == 64 ? k4[u].x : 0u;
-            reg_sort_desc_u32<4>(v);
             uint32_t t = 0u;
+            if (M <= 32) {
+                t = wave_kth_u32<4>(v, M);
+            } else {
+                reg_sort_desc_u32<4>(v);
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                if (r == (M - 1) >> 6) t = v[r];
-            t = (uint32_t)__shfl((int)t, (M - 1) & 63);
+                for (int r = 0; r < 4; ++r)
+                    if (r == (M - 1) >> 6) t = v[r];
+                t = (uint32_t)__shfl((int)t, (M - 1) & 63);
+            }
             if (t != 0u) ts = tk_score(t);
         }
         offer_lists<4>(a, cb, k4, nv, s0, 0, ts, q, qx, have_qx);
@@ -507,6 +542,16 @@ __global__ __launch_bounds__(ACC_WG) void batch_topk(DenseArgs a) {
     topk_by_rank<R>(a.resc + (size_t)q * total, total, a.K[q], a.out + (size_t)q * a.out_stride, (uint64_t *)smem, s_nz);
 }
 
+// The same for K <= 256 (total <= V * ACC_WG): radix selection of the K-th
+// key, then ranks by counting.
+template <int V>
+__global__ __launch_bounds__(ACC_WG) void batch_topk_radix(DenseArgs a) {
+    __shared__ __align__(16) uint32_t lds[1032];
+    const int q = blockIdx.x;
+    const int total = a.split * a.M;
+    topk_radix<V>(a.resc + (size_t)q * total, total, a.K[q], a.out + (size_t)q * a.out_stride, lds);
+}
+
 dense_fn get_dense_build() { return dense_build; }
 dense_fn get_dense_prep() { return dense_prep; }
 // STREAM: non-temporal tile loads, for matrices larger than the last-level
@@ -522,6 +567,16 @@ dense_fn get_dense_scores(int NT, bool stream) {
 }
 dense_fn get_dense_select() { return dense_select; }
 dense_fn get_batch_rescore() { return batch_rescore; }
+dense_fn get_batch_topk_radix(int V) {
+    switch (V) {
+        case 1: return batch_topk_radix<1>;
+        case 2: return batch_topk_radix<2>;
+        case 4: return batch_topk_radix<4>;
+        case 8: return batch_topk_radix<8>;
+        case 16: return batch_topk_radix<16>;
+        default: return nullptr;
+    }
+}
 dense_fn get_batch_topk(int R) {
     switch (R) {
         case 1: return batch_topk<1>;

@@ -589,4 +589,5 @@ dense_fn get_dense_prep();
 dense_fn get_dense_scores(int NT, bool stream);
 dense_fn get_dense_select();
 dense_fn get_batch_rescore();
+dense_fn get_batch_topk_radix(int V);
 dense_fn get_batch_topk(int R);
