@@ -70,6 +70,8 @@ struct acc_gpu_matrix {
     uint64_t *d_sthr = nullptr;
     uint32_t *d_scnt = nullptr;
     uint64_t *d_scand = nullptr;
+    uint64_t *d_rsel = nullptr;      // radix Top-K passes (K <= 256)
+    size_t rsel_cap = 0;
     // low-bit sliced packets
     bool has_packed = false;
     int bits = 8, col_bits = 10, entry_bits = 18, is_signed = 0;
@@ -491,10 +493,54 @@ int check_lowbit(acc_gpu_matrix *m, int ck, int k) {
     return ACC_OK;
 }
 
-// Top-K of a device score vector (select_blocks, then select_final); the
-// state must be zero on entry and is left zero.
+// Keys the radix Top-K passes keep over n scores: every pass leaves K per
+// SEL_CHUNK of its input, until at most 16384 remain (one final workgroup).
+size_t radix_select_keys(int64_t n, int K) {
+    size_t total = 0;
+    int64_t cnt = n;
+    do {
+        const int64_t blocks = std::max<int64_t>(1, (cnt + SEL_CHUNK - 1) / SEL_CHUNK);
+        cnt = blocks * K;
+        total += (size_t)cnt;
+    } while (cnt > 16384);
+    return total;
+}
+
+// Radix Top-K (K <= 256) of n scores; buf holds radix_select_keys(n, K) keys.
+int launch_select_radix(const float *y, int64_t n, int K, uint64_t *buf, acc_element *out, hipStream_t st,
+                        acc_gpu_matrix *prof) {
+    const void *in = y;
+    bool keys = false;
+    int64_t cnt = n;
+    uint64_t *dst = buf;
+    do {
+        const int64_t blocks = std::max<int64_t>(1, (cnt + SEL_CHUNK - 1) / SEL_CHUNK);
+        radix_pass_fn f = get_select_radix_pass(keys);
+        if (!keys) {
+            LAUNCH(prof, 1, f, dim3((unsigned)blocks), dim3(ACC_WG), 0, st, in, cnt, K, dst);
+        } else {
+            hipLaunchKernelGGL(f, dim3((unsigned)blocks), dim3(ACC_WG), 0, st, in, cnt, K, dst);
+        }
+        int rc = check_launch();
+        if (rc != ACC_OK) return rc;
+        in = dst;
+        keys = true;
+        cnt = blocks * K;
+        dst += cnt;
+    } while (cnt > 16384);
+    radix_final_fn ff = get_select_radix_final(pow2_at_least((int)((cnt + 1023) / 1024)));
+    if (!ff) return fail(ACC_ERR_ARG, "too many candidates");
+    LAUNCH(prof, 2, ff, dim3(1), dim3(1024), 0, st, (const uint64_t *)in, (int)cnt, K, out);
+    return check_launch();
+}
+
+// Top-K of a device score vector: radix passes when K <= 256 and rbuf holds
+// radix_select_keys(n, K) keys, else select_blocks then select_final (their
+// state must be zero on entry and is left zero).
 int launch_select(const float *y, int n, int K, int blocks, int chunk, uint64_t *thr, uint32_t *cnt, uint64_t *cand,
-                  acc_element *out, hipStream_t st, acc_gpu_matrix *prof = nullptr) {
+                  acc_element *out, hipStream_t st, acc_gpu_matrix *prof = nullptr, uint64_t *rbuf = nullptr,
+                  size_t rcap = 0) {
+    if (K <= 256 && rbuf && radix_select_keys(n, K) <= rcap) return launch_select_radix(y, n, K, rbuf, out, st, prof);
     SelectArgs s;
     s.y = y;
     s.n = n;
@@ -667,6 +713,8 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         TRY(dalloc(m, &m->d_sthr, 1));
         TRY(dalloc(m, &m->d_scnt, 1));
         TRY(dalloc(m, &m->d_scand, (size_t)m->sel_blocks * kMaxK));
+        m->rsel_cap = radix_select_keys(m->rows, 256);
+        TRY(dalloc(m, &m->d_rsel, m->rsel_cap));
         ZERO(m->d_sthr, 8);
         ZERO(m->d_scnt, 4);
     }
@@ -929,7 +977,7 @@ extern "C" int acc_gpu_topk_fp32(acc_gpu_matrix *m, const float *x_dev, int k, a
     int rc = launch_spmv_fp32(m, x_dev, m->d_y, st);
     if (rc != ACC_OK) return rc;
     return launch_select(m->d_y, m->rows, k, m->sel_blocks, m->sel_chunk, m->d_sthr, m->d_scnt, m->d_scand,
-                         out_dev, st, m);
+                         out_dev, st, m, m->d_rsel, m->rsel_cap);
 }
 
 extern "C" int acc_gpu_set_profiling(acc_gpu_matrix *m, int enable) {
@@ -996,12 +1044,16 @@ extern "C" int acc_gpu_top_k_heap(const float *y_dev, int n, int k, acc_element 
     HIP_TRY(hipMallocAsync((void **)&thr, 8, st));
     HIP_TRY(hipMallocAsync((void **)&cnt, 4, st));
     HIP_TRY(hipMallocAsync((void **)&cand, (size_t)blocks * k * 8, st));
+    uint64_t *rbuf = nullptr;
+    const size_t rcap = k <= 256 ? radix_select_keys(n, k) : 0;
+    if (rcap) HIP_TRY(hipMallocAsync((void **)&rbuf, rcap * 8, st));
     hipLaunchKernelGGL(get_select_reset(), dim3(1), dim3(64), 0, st, thr, cnt);
     int rc = check_launch();
-    if (rc == ACC_OK) rc = launch_select(y_dev, n, k, blocks, chunk, thr, cnt, cand, out_dev, st);
+    if (rc == ACC_OK) rc = launch_select(y_dev, n, k, blocks, chunk, thr, cnt, cand, out_dev, st, nullptr, rbuf, rcap);
     hipFreeAsync(thr, st);
     hipFreeAsync(cnt, st);
     hipFreeAsync(cand, st);
+    if (rbuf) hipFreeAsync(rbuf, st);
     return rc;
 }
 
@@ -1132,7 +1184,7 @@ static int plan_enqueue(acc_plan *p, bool upload_k, hipStream_t st) {
             int rc = launch_spmv_fp32(m, p->x, m->d_y, st);
             if (rc != ACC_OK) return rc;
             return launch_select(m->d_y, m->rows, p->kmax, m->sel_blocks, m->sel_chunk, m->d_sthr, m->d_scnt,
-                                 m->d_scand, p->out, st);
+                                 m->d_scand, p->out, st, nullptr, m->d_rsel, m->rsel_cap);
         }
         default:
             return fail(ACC_ERR_ARG, "unknown plan kind");

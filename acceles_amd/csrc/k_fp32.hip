@@ -1,6 +1,7 @@
 // k_fp32.hip -- plain FP32 CSR path and FP64 exact scores:
 //   fp32_spmv       FP32 CSR SpMV (cpu_mkl_baseline.cpp:261);
-//   select_blocks / select_final
+//   select_radix_pass / select_radix_final (K <= 256),
+//   select_blocks / select_final (any K <= 960)
 //                   Top-K of a score vector (cpu_mkl_baseline.cpp:128-149);
 //                   with fp32_spmv it is the plain FP32 CSR Top-K path
 //                   (host_spmv_topk_csr_gpu.cu:171-231 is the GPU spec);
@@ -129,6 +130,55 @@ __global__ __launch_bounds__(ACC_WG) void select_final(SelectArgs a) {
     }
 }
 
+// Radix Top-K (K <= 256).  Pass: workgroup b keeps the K best keys of
+// elements [b*SEL_CHUNK, (b+1)*SEL_CHUNK) of the input (FP32 scores keyed as
+// (score, row i), or keys of an earlier pass) by radix selection and writes
+// them to cand[b*K, (b+1)*K), zero-padded.  Passes repeat until the survivors
+// fit one workgroup; then select_radix_final ranks them.
+template <bool KEYS>
+__global__ __launch_bounds__(ACC_WG) void select_radix_pass(const void *in, int64_t n, int K, uint64_t *cand) {
+    constexpr int V = SEL_CHUNK / ACC_WG;
+    __shared__ uint32_t hist[512];
+    __shared__ uint64_t s_t[2];
+    __shared__ int s_need[2], s_n;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int64_t b0 = (int64_t)blockIdx.x * SEL_CHUNK;
+    uint64_t k[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+        const int64_t i = b0 + tid + u * ACC_WG;
+        if (KEYS) k[u] = i < n ? ((const uint64_t *)in)[i] : 0ull;
+        else k[u] = i < n ? dkey_f32(__builtin_nontemporal_load((const float *)in + i), (int32_t)i) : 0ull;
+    }
+    if (tid == 0) s_n = 0;
+    const uint64_t T = wg_radix_kth<uint64_t>(
+        [&](auto fn) {
+#pragma unroll
+            for (int u = 0; u < V; ++u) fn(k[u]);
+        },
+        K, hist, s_t, s_need);
+    uint64_t *dst = cand + (size_t)blockIdx.x * K;
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+        const bool sel = k[u] != 0ull && k[u] >= T;
+        const uint64_t m = __ballot(sel);
+        if (m == 0) continue;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&s_n, __popcll(m));
+        base = __shfl(base, 0);
+        const int j = base + mbcnt64(m);
+        if (sel && j < K) dst[j] = k[u];  // j < K for distinct keys
+    }
+    __syncthreads();
+    for (int j = min(s_n, K) + tid; j < K; j += ACC_WG) dst[j] = 0ull;
+}
+
+template <int V>
+__global__ __launch_bounds__(1024) void select_radix_final(const uint64_t *cand, int total, int K, acc_element *out) {
+    __shared__ __align__(16) uint32_t lds[1032];
+    topk_radix<V>(cand, total, K, out, lds);
+}
+
 // Zeroes the select state (a kernel rather than a memset, so the launch
 // sequence can be captured in a graph).
 __global__ void select_reset(uint64_t *thr, uint32_t *cnt) {
@@ -182,4 +232,15 @@ select_fn get_select_final(int R) {
     }
 }
 reset_fn get_select_reset() { return select_reset; }
+radix_pass_fn get_select_radix_pass(bool keys) { return keys ? select_radix_pass<true> : select_radix_pass<false>; }
+radix_final_fn get_select_radix_final(int V) {
+    switch (V) {
+        case 1: return select_radix_final<1>;
+        case 2: return select_radix_final<2>;
+        case 4: return select_radix_final<4>;
+        case 8: return select_radix_final<8>;
+        case 16: return select_radix_final<16>;
+        default: return nullptr;
+    }
+}
 fp64_fn get_spmv_fp64() { return spmv_fp64; }

@@ -30,6 +30,7 @@
 #define LB_WAVES 8
 #define LB_WG (LB_WAVES * 64)
 #define LB_MAX_ITEM 1024  // slices per stage-1 work item (their metadata sits in LDS)
+#define SEL_CHUNK 8192  // elements per workgroup of a radix selection pass
 #define SEL_WAVES 16
 #define SEL_WG (SEL_WAVES * 64)
 
@@ -567,6 +568,8 @@ typedef void (*select_fn)(SelectArgs);
 typedef void (*batch_fn)(BatchArgs);
 typedef void (*spmv_fn)(const uint32_t *, const int32_t *, const float *, const float *, int, int, float *);
 typedef void (*reset_fn)(uint64_t *, uint32_t *);
+typedef void (*radix_pass_fn)(const void *, int64_t, int, uint64_t *);
+typedef void (*radix_final_fn)(const uint64_t *, int, int, acc_element *);
 typedef void (*fp64_fn)(const uint32_t *, const int32_t *, const float *, const float *, int, double *);
 
 lowbit_fn get_lowbit_slicemax(int E, bool is_signed);  // k_lowbit.hip
@@ -577,6 +580,8 @@ scores_fn get_lowbit_scores(int E, bool is_signed);
 spmv_fn get_fp32_spmv(int G);                         // k_fp32.hip
 select_fn get_select_blocks(int R);
 select_fn get_select_final(int R);
+radix_pass_fn get_select_radix_pass(bool keys);  // k_fp32.hip
+radix_final_fn get_select_radix_final(int V);
 reset_fn get_select_reset();
 fp64_fn get_spmv_fp64();
 batch_fn get_batch_scores16(int E, bool is_signed, int bits);  // k_batch16.hip

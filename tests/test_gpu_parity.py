@@ -90,16 +90,21 @@ def test_fp32_topk_more_than_rows(acc, gpu):
     np.testing.assert_array_equal(out["index"], oi)
 
 
-def test_dense_top_k_heap(acc, gpu):
-    rng = np.random.default_rng(0)
-    y = rng.integers(-1000, 1000, 300000).astype(np.float32)  # many ties: lower row must win
-    out = empty(acc, 200, np.int32)
-    acc.top_k_heap(dev(acc, y), len(y), 100, out)
+@pytest.mark.parametrize("n,k", [(300000, 100), (300000, 300), (2500000, 256), (5000, 7), (0, 5)])
+def test_dense_top_k_heap(acc, gpu, n, k):
+    # radix passes for k <= 256 (2.5M: a second pass over the first pass's keys), sorted lists above
+    rng = np.random.default_rng(n + k)
+    y = rng.integers(-1000, 1000, n).astype(np.float32)  # many ties: lower row must win
+    y[:: 7] = -0.0
+    out = empty(acc, 2 * k, np.int32)
+    acc.top_k_heap(dev(acc, y) if n else empty(acc, 1, np.float32), n, k, out)
     acc.synchronize()
     v, i = els(out)
-    oi, ov = O.topk(y, 100)
-    np.testing.assert_array_equal(i, oi)
-    np.testing.assert_array_equal(v, ov)
+    oi, ov = O.topk(y, k)
+    m = min(n, k)
+    np.testing.assert_array_equal(i[:m], oi[:m])
+    np.testing.assert_array_equal(v[:m], ov[:m])
+    assert (i[m:] == -1).all() and np.isneginf(v[m:]).all()
 
 
 # --------------------------------------------------------------------------
