@@ -71,6 +71,37 @@ __device__ __forceinline__ int32_t wave_max_lane63(int32_t v) {
     return v;
 }
 
+// The m-th largest (m >= 1, with repeats) of the U values per lane of the
+// wavefront (0 = no value; returns 0 when fewer than m are nonzero), by
+// sorting each lane's values in registers and then extracting the wavefront
+// maximum m times: about 15 instructions per extracted value, against the 36
+// shuffle stages of a 256-key bitonic sort.
+template <int U>
+__device__ __forceinline__ uint32_t wave_kth_u32(uint32_t (&v)[U], int m) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+#pragma unroll
+        for (int j = 0; j + 1 < U - i; ++j) {
+            const uint32_t x = v[j], y = v[j + 1];
+            v[j] = max(x, y);
+            v[j + 1] = min(x, y);
+        }
+    uint32_t t = 0u;
+    for (int it = 0; it < m; ++it) {
+        const uint32_t mx = wave_umax(v[0]);
+        if (mx == 0u) return 0u;
+        const int l = __builtin_ctzll(__ballot(v[0] == mx));
+        if (lane == l) {
+#pragma unroll
+            for (int j = 0; j + 1 < U; ++j) v[j] = v[j + 1];
+            v[U - 1] = 0u;
+        }
+        t = mx;
+    }
+    return t;
+}
+
 __device__ __forceinline__ int mbcnt64(uint64_t mask) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }

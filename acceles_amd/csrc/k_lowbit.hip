@@ -127,7 +127,7 @@ __global__ __launch_bounds__(SEL_WG) void lowbit_select(LowbitArgs a) {
     constexpr int V = 8;         // slice maxima per thread held in registers
     constexpr int kList = 2048;  // candidate slices listed in LDS
     constexpr int CS = 32;       // candidate slices per rescan chunk
-    constexpr int kU = E <= 17 ? 3 : 2;  // groups in flight per wavefront (128 VGPRs at 16 waves)
+    constexpr int kU = E <= 16 ? 3 : 2;  // groups in flight per wavefront (128 VGPRs at 16 waves)
     constexpr int KCAP = 4096;   // LDS key list (rows >= ts)
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t s_hist[512];
@@ -168,29 +168,50 @@ __global__ __launch_bounds__(SEL_WG) void lowbit_select(LowbitArgs a) {
     }
     quantize_query_lds(a.x, a.cols, qx, red);
 
-    // 1. threshold (unsigned order: score ^ 2^31; key 0 = not a full slice)
+    // 1. threshold (unsigned order: score ^ 2^31; key 0 = not a full slice):
+    //    up to 256 slices and M <= 32 by maximum extraction in wavefront 0,
+    //    otherwise by radix selection over the workgroup
     auto mkey = [](int32_t v, bool ok) -> uint32_t { return ok ? ((uint32_t)v ^ 0x80000000u) : 0u; };
-    const uint32_t T = wg_radix_kth<uint32_t>(
-        [&](auto fn) {
-            if (regs) {
+    uint32_t T;
+    if (nsl <= 256 && a.M <= 32) {
+        if (tid < 256) s_hist[tid] = tid < nsl ? mkey(sm[0], full[0]) : 0u;  // regs holds slice tid in sm[0]
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t v[4];
 #pragma unroll
-                for (int u = 0; u < V; ++u) fn(tid + u * SEL_WG < nsl ? mkey(sm[u], full[u]) : 0u);
-            } else {
-                for (int i = tid; i < nsl; i += SEL_WG) fn(mkey(smax[i], nvalid[i] == 64));
-            }
-        },
-        a.M, s_hist, s_t32, s_need);
+            for (int u = 0; u < 4; ++u) v[u] = s_hist[u * 64 + lane];
+            const uint32_t t = wave_kth_u32<4>(v, a.M);
+            if (lane == 0) s_t32[0] = t;
+        }
+        __syncthreads();
+        T = s_t32[0];
+    } else {
+        T = wg_radix_kth<uint32_t>(
+            [&](auto fn) {
+                if (regs) {
+#pragma unroll
+                    for (int u = 0; u < V; ++u) fn(tid + u * SEL_WG < nsl ? mkey(sm[u], full[u]) : 0u);
+                } else {
+                    for (int i = tid; i < nsl; i += SEL_WG) fn(mkey(smax[i], nvalid[i] == 64));
+                }
+            },
+            a.M, s_hist, s_t32, s_need);
+    }
     const int32_t ts = T == 0u ? INT32_MIN : (int32_t)(T ^ 0x80000000u);
 
-    // 2. candidate slices
+    // 2. candidate slices, listed with their metadata
     auto offer = [&](int i, bool cand) {
         const uint64_t m = __ballot(cand);
         if (m == 0) return;
+        const uint2 meta = cand ? a.slices[s0 + i] : make_uint2(0u, 0u);  // in flight during the append
         int base = 0;
         if (lane == 0) base = atomicAdd(&s_cnt, __popcll(m));
         base = __shfl(base, 0);
         const int j = base + mbcnt64(m);
-        if (cand && j < kList) s_list[j] = i;
+        if (cand && j < kList) {
+            s_list[j] = i;
+            s_lmeta[j] = meta;
+        }
     };
     if (regs) {
 #pragma unroll
@@ -233,7 +254,6 @@ __global__ __launch_bounds__(SEL_WG) void lowbit_select(LowbitArgs a) {
             if (tid == 0) s_tn = 0;
             return m;
         };
-        for (int j = tid; j < ncand; j += SEL_WG) s_lmeta[j] = a.slices[s0 + s_list[j]];
         for (int cb = 0; cb < ncand; cb += CS) {
             const int cn = min(CS, ncand - cb);
             __syncthreads();  // metadata listed; the previous chunk's sums and keys in place
@@ -310,12 +330,22 @@ __global__ __launch_bounds__(SEL_WG) void lowbit_select(LowbitArgs a) {
             }
         }
         __syncthreads();
-        mcount = cut(s_kn);
-        if (tid < mcount) {  // rank by counting (keys are distinct)
-            const uint64_t k = s_tmp[tid];
-            int rank = 0;
-            for (int j = 0; j < mcount; ++j) rank += s_tmp[j] > k;
-            s_top[rank] = k;
+        const int nk = s_kn;
+        if (nk <= 64) {  // one wavefront sorts the list (keys are distinct; 0 sorts last)
+            if (wave == 0) {
+                uint64_t w[1] = {lane < nk ? keys[lane] : 0ull};
+                reg_sort_desc<1>(w);
+                if (lane < a.M) s_top[lane] = w[0];
+            }
+            mcount = min(nk, a.M);
+        } else {
+            mcount = cut(nk);
+            if (tid < mcount) {  // rank by counting (keys are distinct)
+                const uint64_t k = s_tmp[tid];
+                int rank = 0;
+                for (int j = 0; j < mcount; ++j) rank += s_tmp[j] > k;
+                s_top[rank] = k;
+            }
         }
         __syncthreads();
     } else {
