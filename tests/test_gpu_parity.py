@@ -122,6 +122,7 @@ CASES = [
     (64, 100, (1, 10), 8, 64, False, 0),        # one row per partition
     (600000, 64, (2, 8), 8, 1, False, 0),       # one partition of 9375 slices (maxima re-read per digit)
     (200000, 4, (1, 4), 2, 1, False, 0),        # 2-bit, 4 columns: thousands of slices tie at the top
+    (60000, 4, (1, 1), 2, 1, False, 0),         # one entry per row: thousands of tied rows in listed slices
 ]
 
 
