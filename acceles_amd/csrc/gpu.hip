@@ -493,13 +493,20 @@ int check_lowbit(acc_gpu_matrix *m, int ck, int k) {
     return ACC_OK;
 }
 
+// Elements per thread of a radix Top-K pass over cnt inputs: 8 (2048 per
+// workgroup) up to 2^21 inputs, so that small vectors still spread over the
+// chip, else 32 (8192 per workgroup).
+int radix_pass_v(int64_t cnt) { return cnt <= (int64_t(1) << 21) ? 8 : 32; }
+
 // Keys the radix Top-K passes keep over n scores: every pass leaves K per
-// SEL_CHUNK of its input, until at most 16384 remain (one final workgroup).
+// workgroup chunk of its input, until at most 16384 remain (one final
+// workgroup).
 size_t radix_select_keys(int64_t n, int K) {
     size_t total = 0;
     int64_t cnt = n;
     do {
-        const int64_t blocks = std::max<int64_t>(1, (cnt + SEL_CHUNK - 1) / SEL_CHUNK);
+        const int64_t chunk = (int64_t)radix_pass_v(cnt) * ACC_WG;
+        const int64_t blocks = std::max<int64_t>(1, (cnt + chunk - 1) / chunk);
         cnt = blocks * K;
         total += (size_t)cnt;
     } while (cnt > 16384);
@@ -514,8 +521,10 @@ int launch_select_radix(const float *y, int64_t n, int K, uint64_t *buf, acc_ele
     int64_t cnt = n;
     uint64_t *dst = buf;
     do {
-        const int64_t blocks = std::max<int64_t>(1, (cnt + SEL_CHUNK - 1) / SEL_CHUNK);
-        radix_pass_fn f = get_select_radix_pass(keys);
+        const int V = radix_pass_v(cnt);
+        const int64_t chunk = (int64_t)V * ACC_WG;
+        const int64_t blocks = std::max<int64_t>(1, (cnt + chunk - 1) / chunk);
+        radix_pass_fn f = get_select_radix_pass(keys, V);
         if (!keys) {
             LAUNCH(prof, 1, f, dim3((unsigned)blocks), dim3(ACC_WG), 0, st, in, cnt, K, dst);
         } else {

@@ -131,18 +131,17 @@ __global__ __launch_bounds__(ACC_WG) void select_final(SelectArgs a) {
 }
 
 // Radix Top-K (K <= 256).  Pass: workgroup b keeps the K best keys of
-// elements [b*SEL_CHUNK, (b+1)*SEL_CHUNK) of the input (FP32 scores keyed as
+// elements [b*V*ACC_WG, (b+1)*V*ACC_WG) of the input (FP32 scores keyed as
 // (score, row i), or keys of an earlier pass) by radix selection and writes
 // them to cand[b*K, (b+1)*K), zero-padded.  Passes repeat until the survivors
 // fit one workgroup; then select_radix_final ranks them.
-template <bool KEYS>
+template <bool KEYS, int V>
 __global__ __launch_bounds__(ACC_WG) void select_radix_pass(const void *in, int64_t n, int K, uint64_t *cand) {
-    constexpr int V = SEL_CHUNK / ACC_WG;
     __shared__ uint32_t hist[512];
     __shared__ uint64_t s_t[2];
     __shared__ int s_need[2], s_n;
     const int tid = threadIdx.x, lane = tid & 63;
-    const int64_t b0 = (int64_t)blockIdx.x * SEL_CHUNK;
+    const int64_t b0 = (int64_t)blockIdx.x * (V * ACC_WG);
     uint64_t k[V];
 #pragma unroll
     for (int u = 0; u < V; ++u) {
@@ -232,7 +231,11 @@ select_fn get_select_final(int R) {
     }
 }
 reset_fn get_select_reset() { return select_reset; }
-radix_pass_fn get_select_radix_pass(bool keys) { return keys ? select_radix_pass<true> : select_radix_pass<false>; }
+radix_pass_fn get_select_radix_pass(bool keys, int V) {
+    if (V == 8) return keys ? select_radix_pass<true, 8> : select_radix_pass<false, 8>;
+    if (V == 32) return keys ? select_radix_pass<true, 32> : select_radix_pass<false, 32>;
+    return nullptr;
+}
 radix_final_fn get_select_radix_final(int V) {
     switch (V) {
         case 1: return select_radix_final<1>;

@@ -30,7 +30,6 @@
 #define LB_WAVES 8
 #define LB_WG (LB_WAVES * 64)
 #define LB_MAX_ITEM 1024  // slices per stage-1 work item (their metadata sits in LDS)
-#define SEL_CHUNK 8192  // elements per workgroup of a radix selection pass
 #define SEL_WAVES 16
 #define SEL_WG (SEL_WAVES * 64)
 
@@ -580,7 +579,7 @@ scores_fn get_lowbit_scores(int E, bool is_signed);
 spmv_fn get_fp32_spmv(int G);                         // k_fp32.hip
 select_fn get_select_blocks(int R);
 select_fn get_select_final(int R);
-radix_pass_fn get_select_radix_pass(bool keys);  // k_fp32.hip
+radix_pass_fn get_select_radix_pass(bool keys, int V);  // k_fp32.hip
 radix_final_fn get_select_radix_final(int V);
 reset_fn get_select_reset();
 fp64_fn get_spmv_fp64();
