@@ -449,6 +449,10 @@ struct ScoresArgs {
 };
 
 
+// Top-K of y[0, n) for any K <= 960 (select_blocks: workgroup b scans
+// [b*chunk, (b+1)*chunk), each wavefront a contiguous quarter, with
+// wavefront collectors, merges them and appends what can still be in the
+// Top-K; select_final ranks the appended keys).
 struct SelectArgs {
     const float *y;
     int n, chunk, K;
@@ -457,10 +461,6 @@ struct SelectArgs {
     uint64_t *cand;  // gridDim.x * K
     acc_element *out;
 };
-
-// Top-K of y[0, n), first launch: workgroup b scans [b*chunk, (b+1)*chunk)
-// (each wavefront a contiguous quarter) with wavefront collectors, merges
-// them and appends what can still be in the Top-K.
 
 // ---------------------------------------------------------------------------
 // batched low-bit path: a block of QB queries per matrix pass
