@@ -52,6 +52,12 @@ struct BatchWs {
     size_t tk_n = 0;
     uint64_t *ckeys = nullptr;   // per (query, partition) candidates
     size_t ckeys_n = 0;
+    int32_t *smax = nullptr;     // single-query passes: slice maxima
+    size_t smax_n = 0;
+    float *y = nullptr;          // FP32 plans: scores
+    size_t y_n = 0;
+    uint64_t *rsel = nullptr;    // FP32 plans: radix selection keys
+    size_t rsel_n = 0;
 };
 
 struct acc_gpu_matrix {
@@ -234,7 +240,7 @@ int lds_limit(const void *fn, size_t bytes);
 // Top-K into out (K > 0) and/or the per-partition candidates into
 // crow/cscore (p*M + i).
 LowbitArgs lowbit_args(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *out, int32_t *crow,
-                       int32_t *cscore) {
+                       int32_t *cscore, int32_t *smax = nullptr, uint64_t *resc = nullptr) {
     LowbitArgs a;
     a.words = m->d_words;
     a.slices = m->d_slices;
@@ -242,7 +248,7 @@ LowbitArgs lowbit_args(acc_gpu_matrix *m, const float *x, int M, int K, acc_elem
     a.items = m->d_items;
     a.part_slice = m->d_part_slice;
     a.nvalid = m->d_nvalid;
-    a.smax = m->d_smax;
+    a.smax = smax ? smax : m->d_smax;
     a.x = x;
     a.cols = m->cols;
     a.col_bits = m->col_bits;
@@ -253,7 +259,7 @@ LowbitArgs lowbit_args(acc_gpu_matrix *m, const float *x, int M, int K, acc_elem
     a.idx = m->d_idx;
     a.val = m->d_val;
     a.G = m->group;
-    a.resc = m->d_resc;
+    a.resc = resc ? resc : m->d_resc;
     a.K = K;
     a.out = out;
     a.cand_rows_out = crow;
@@ -287,9 +293,16 @@ int launch_lowbit_stage(acc_gpu_matrix *m, const LowbitArgs &a, int stage, hipSt
     return check_launch();
 }
 
+// ws: a plan's or a batch's workspace for the slice maxima and rescored keys
+// (null: the handle's own).
 int launch_lowbit(acc_gpu_matrix *m, const float *x, int M, int K, acc_element *out, int32_t *crow, int32_t *cscore,
-                  hipStream_t st) {
-    const LowbitArgs a = lowbit_args(m, x, M, K, out, crow, cscore);
+                  hipStream_t st, BatchWs *ws = nullptr) {
+    if (ws) {
+        int rc = ws_grow(*ws, &ws->smax, &ws->smax_n, (size_t)m->nslices);
+        if (rc == ACC_OK) rc = ws_grow(*ws, &ws->resc, &ws->resc_n, (size_t)m->split * kMaxM);
+        if (rc != ACC_OK) return rc;
+    }
+    const LowbitArgs a = lowbit_args(m, x, M, K, out, crow, cscore, ws ? ws->smax : nullptr, ws ? ws->resc : nullptr);
     int rc = launch_lowbit_stage(m, a, 0, st);
     if (rc == ACC_OK) rc = launch_lowbit_stage(m, a, 1, st);
     if (rc == ACC_OK && K > 0) rc = launch_lowbit_stage(m, a, 2, st);
@@ -872,7 +885,7 @@ static int run_batch(acc_gpu_matrix *m, BatchWs &ws, const float *x_dev, int nq,
         int rc = ACC_OK;
         for (int q = 0; q < nq && rc == ACC_OK; ++q)
             rc = launch_lowbit(m, x_dev + (size_t)q * m->cols, M, k_host[q], out_dev + (size_t)q * out_stride, nullptr,
-                               nullptr, st);
+                               nullptr, st, &ws);
         return rc;
     }
     int rc = ws_grow(ws, &ws.K, &ws.K_n, (size_t)nq);
@@ -1186,14 +1199,17 @@ static int plan_enqueue(acc_plan *p, bool upload_k, hipStream_t st) {
     acc_gpu_matrix *m = p->m;
     switch (p->kind) {
         case ACC_PLAN_LOWBIT:
-            return launch_lowbit(m, p->x, p->ck / m->split, p->kmax, p->out, nullptr, nullptr, st);
+            return launch_lowbit(m, p->x, p->ck / m->split, p->kmax, p->out, nullptr, nullptr, st, &p->ws);
         case ACC_PLAN_LOWBIT_BATCH:
             return run_batch(m, p->ws, p->x, p->nq, p->k.data(), upload_k, p->kmax, p->ck, p->out, p->out_stride, st);
         case ACC_PLAN_FP32: {
-            int rc = launch_spmv_fp32(m, p->x, m->d_y, st);
+            const size_t rcap = radix_select_keys(m->rows, 256);
+            int rc = ws_grow(p->ws, &p->ws.y, &p->ws.y_n, (size_t)m->rows);
+            if (rc == ACC_OK) rc = ws_grow(p->ws, &p->ws.rsel, &p->ws.rsel_n, rcap);
+            if (rc == ACC_OK) rc = launch_spmv_fp32(m, p->x, p->ws.y, st);
             if (rc != ACC_OK) return rc;
-            return launch_select(m->d_y, m->rows, p->kmax, m->sel_blocks, m->sel_chunk, m->d_sthr, m->d_scnt,
-                                 m->d_scand, p->out, st, nullptr, m->d_rsel, m->rsel_cap);
+            return launch_select(p->ws.y, m->rows, p->kmax, m->sel_blocks, m->sel_chunk, m->d_sthr, m->d_scnt,
+                                 m->d_scand, p->out, st, nullptr, p->ws.rsel, rcap);
         }
         default:
             return fail(ACC_ERR_ARG, "unknown plan kind");

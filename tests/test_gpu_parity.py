@@ -390,6 +390,34 @@ def test_plan_replays_match_eager(acc, gpu, kind, nq, mode):
     plan.close()
 
 
+@pytest.mark.parametrize("kind", ["lowbit", "fp32"])
+def test_plans_in_flight_on_two_streams(acc, gpu, kind):
+    # plans own their query workspaces: two plans of one handle run concurrently on two streams
+    csr, pk, g = make(acc, 60000, 500, (40, 100), 8, 32, seed=78)
+    K, CK = 100, 512
+    pkind = {"lowbit": acc.PLAN_LOWBIT, "fp32": acc.PLAN_FP32}[kind]
+    X = acc.generate_queries(2, 500, seed=4242)
+    xs = [dev(acc, X[j]) for j in range(2)]
+    outs = [empty(acc, K * 2, np.int32) for _ in range(2)]
+    plans = [acc.Plan(g, pkind, xs[j], [K], CK, outs[j], K) for j in range(2)]
+    streams = [acc.Stream(0), acc.Stream(0)]
+    for rep in range(6):
+        for j in range(2):
+            plans[j].launch(streams[j])
+    for s in streams:
+        s.synchronize()
+    ref = empty(acc, K * 2, np.int32)
+    for j in range(2):
+        if kind == "lowbit":
+            g.topk_lowbit(xs[j], K, CK, ref)
+        else:
+            g.topk_fp32(xs[j], K, ref)
+        acc.synchronize()
+        np.testing.assert_array_equal(outs[j].numpy(), ref.numpy(), err_msg=f"plan {j}")
+    for p in plans:
+        p.close()
+
+
 def test_autotune_keeps_results(acc, gpu):
     """The tuned layout choice per batch size changes speed, not results."""
     csr, pk, g = make(acc, 20000, 500, (40, 100), 8, 32, seed=91)
