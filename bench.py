@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--inflight", type=int, default=2, choices=[1, 2, 3, 4],
+                    help="batches (embedding) or queries (large-single) in flight on as many streams")
     ap.add_argument("--batch-mode", choices=["auto", "packets", "dense"], default="auto",
                     help="scoring layout of the batched path")
     ap.add_argument("--workload", choices=["embedding", "large-single", "batched", "serve", "copy"],
@@ -79,13 +81,19 @@ def cpu_baseline(csr, X, k, seconds):
 
 
 def timed(stream, fn, steps, warmup):
+    return timed_multi([stream], fn, steps, warmup)
+
+
+def timed_multi(streams, fn, steps, warmup):
     for _ in range(warmup):
         fn()
-    stream.synchronize()
+    for s in streams:
+        s.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         fn()
-    stream.synchronize()
+    for s in streams:
+        s.synchronize()
     return time.perf_counter() - t0
 
 
@@ -145,25 +153,17 @@ def bench_large_single(a, acc, stream):
         g.topk_lowbit(xs[i[0] % nq], K, CK, os_[i[0] % nq], stream)
         i[0] += 1
     dt_seq = timed(stream, step, a.steps, a.warmup)
-    # two queries in flight: HIP-graph plans with their own workspaces on two streams, so one query's
+    # queries in flight: HIP-graph plans with their own workspaces on as many streams, so one query's
     # selection kernels (a few dozen workgroups) overlap the next query's streaming pass
-    streams = [stream, acc.Stream(0)]
-    plans = [acc.Plan(g, acc.PLAN_LOWBIT, xs[j], [K], CK, os_[j], K) for j in range(2)]
+    NF = a.inflight
+    streams = [stream] + [acc.Stream(0) for _ in range(1, NF)]
+    plans = [acc.Plan(g, acc.PLAN_LOWBIT, xs[j], [K], CK, os_[j], K) for j in range(NF)]
 
     def step2():
-        j = i[0] & 1
+        j = i[0] % NF
         plans[j].launch(streams[j])
         i[0] += 1
-    for _ in range(a.warmup):
-        step2()
-    for s_ in streams:
-        s_.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step2()
-    for s_ in streams:
-        s_.synchronize()
-    dt = time.perf_counter() - t0
+    dt = timed_multi(streams, step2, a.steps, a.warmup)
     for p_ in plans:
         p_.close()
     qps = a.steps / dt
@@ -186,12 +186,12 @@ def bench_large_single(a, acc, stream):
     res = out.numpy()
     prec = [precision_at(res[j, :, 1], exact_topk(g, acc, xs[j], rows, K, stream)) for j in range(min(nq, 8))]
     precision = dict(zip(map(str, KS), np.round(np.mean(prec, axis=0), 5).tolist()))
-    log(f"[bench] single query: {qps:.0f} q/s with two in flight ({a.steps / dt_seq:.0f} q/s one at a time); "
+    log(f"[bench] single query: {qps:.0f} q/s with {NF} in flight ({a.steps / dt_seq:.0f} q/s one at a time); "
         f"slicemax {lb_ms[0] * 1e3:.1f}us ({s1_gbs:.0f} GB/s), select "
         f"{lb_ms[1] * 1e3:.1f}us, final {lb_ms[2] * 1e3:.1f}us; fp32 CSR Top-K {fqps:.0f} q/s; precision {precision}")
     line = line_common(a, "single-query low-bit Top-100 SpMV queries/s", qps, "queries/s", dt * 1e3 / a.steps, "u8",
                        f"single-query low-bit Top-100: {rows}x{cols}, row nnz {a.deg_dist} mean 20, |N(0,1)|, 8-bit "
-                       f"packets, SPLIT=64, CK=1024; one query per step, two in flight (graph plans on two streams)",
+                       f"packets, SPLIT=64, CK=1024; one query per step, {NF} in flight (graph plans on as many streams)",
                        {"rows": rows, "cols": cols, "nnz": csr.nnz})
     line.update({
         "roofline": {"bound": "hbm", "kernel": "lowbit_slicemax", "achieved": round(s1_gbs, 1), "peak": HBM_PEAK_GBS,
@@ -338,12 +338,33 @@ def main():
         for i in range(B):
             g.topk_fp32(xs[i], K, outs1[i], stream)
 
-    # ---- timed region: batched low-bit two-stage ----
-    dt = timed(stream, batch_step, a.steps, a.warmup)
+    # ---- timed region: batched low-bit two-stage, one batch at a time (eager calls) ----
+    dt1 = timed(stream, batch_step, a.steps, a.warmup)
+    qps1 = a.steps * B / dt1
+    res = out.numpy()
+
+    # ---- timed region: two batches in flight (HIP-graph plans with their own workspaces on two streams):
+    # one batch's selection, rescoring and Top-K kernels overlap the next batch's tile pass ----
+    NF = a.inflight
+    Xs = [Xd] + [acc.DeviceBuffer.from_numpy(acc.generate_queries(B, a.cols, seed=0xACCE1E5 + 7 + j))
+                 for j in range(1, NF)]
+    outs2 = [acc.DeviceBuffer((B, K, 2), np.int32) for _ in range(NF)]
+    streams = [stream] + [acc.Stream(0) for _ in range(1, NF)]
+    plans = [acc.Plan(g, acc.PLAN_LOWBIT_BATCH, Xs[j], ks, CK, outs2[j], K) for j in range(NF)]
+    it = [0]
+
+    def pipe_step():
+        j = it[0] % NF
+        plans[j].launch(streams[j])
+        it[0] += 1
+    dt = timed_multi(streams, pipe_step, a.steps, a.warmup)
     qps = a.steps * B / dt
     ms_step = dt * 1e3 / a.steps
-    log(f"[bench] low-bit two-stage, batched: {qps:.0f} queries/s ({ms_step:.3f} ms per {B}-query step)")
-    res = out.numpy()
+    pipelined_same = bool(np.array_equal(outs2[0].numpy(), res))
+    for p_ in plans:
+        p_.close()
+    log(f"[bench] low-bit two-stage, batched: {qps:.0f} queries/s with {NF} batches in flight ({ms_step:.3f} ms per "
+        f"{B}-query step); one batch at a time {qps1:.0f} q/s; in-flight results equal eager: {pipelined_same}")
 
     # ---- the other batched layout, for comparison ----
     other_qps = None
@@ -360,7 +381,7 @@ def main():
     res1 = out1.numpy()
     fqps = s_steps * B / timed(stream, fp32_step, s_steps, 1)
     log(f"[bench] low-bit single-query {single_qps:.0f} q/s; fp32 CSR Top-K {fqps:.0f} q/s; "
-        f"batched/fp32 {qps / fqps:.2f}x")
+        f"batched/fp32 {qps1 / fqps:.2f}x (both one at a time)")
     same = bool(np.array_equal(res, res1))
 
     # ---- per-launch times from HIP events on the query stream ----
@@ -448,7 +469,7 @@ def main():
         "config": {
             "workload": f"low-bit two-stage Top-{K}, batched: {a.rows}x{a.cols}, row nnz U[{a.deg[0]},{a.deg[1]}], "
                         f"|N(0,1)| L2-normalised, {a.bits}-bit packets, SPLIT={a.split}, CK={CK}, "
-                        f"{B} N(0,1) queries per step",
+                        f"{B} N(0,1) queries per step, {a.inflight} steps in flight (graph plans on as many streams)",
             "rows": a.rows, "cols": a.cols, "nnz": csr.nnz, "bits": a.bits, "split": a.split, "ck": CK,
             "k": K, "queries_per_step": B, "parallelism": "single GPU",
         },
@@ -482,9 +503,11 @@ def main():
             "avg_launch_us": round(float(lb_ms[0]) * 1e3, 2),
         },
         "cpu_baseline": cpu,
+        "one_batch_at_a_time_qps": round(qps1, 2),
+        "in_flight_equals_eager": pipelined_same,
         "single_query_qps": round(single_qps, 2),
         "fp32_csr_gpu_qps": round(fqps, 2),
-        "lowbit_batched_vs_fp32_speedup": round(qps / fqps, 3),
+        "lowbit_batched_vs_fp32_speedup": round(qps1 / fqps, 3),  # both one at a time
         "batched_equals_single": same,
         "batch_launch_us": dict(zip(names, [round(float(v) * 1e3, 2) for v in bt_ms])),
         "batch_scoring": "dense int8 tiles (MFMA)" if dense else "sliced packets (dot4)",
