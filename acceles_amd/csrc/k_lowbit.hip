@@ -9,7 +9,8 @@
 //                    maximum reaches it, Top-M (quantify_util.py:173-196,
 //                    host_spmv_ucsr.cpp:691-748) and FP32 rescoring of those
 //                    rows (quantify_util.py:354-367);
-//   lowbit_final     the global Top-K of the rescored candidates;
+//   lowbit_final_radix / lowbit_final
+//                    the global Top-K of the rescored candidates;
 //   lowbit_scores    INT32 scores of every row (parity checks).
 // The streaming pass carries no selection state, so it runs at full
 // occupancy with nothing but loads, decode and one wavefront max per slice.
