@@ -788,9 +788,12 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         m->nkb = m->cols_pad / 32;
         m->u8off = (!m->is_signed && m->bits == 8) ? 1 : 0;
         {
-            const int64_t maxv = m->is_signed ? (1 << (m->bits - 1)) : ((1 << m->bits) - 1);
-            if (pk->dup_rows > 0)
-                m->dense_why = "a row names a column twice (the dense layout holds one value per column)";
+            // a tile byte holds the sum of a row's values in one column (rows
+            // may name a column twice): v - 128 for unsigned 8-bit, else v
+            const int64_t lo = m->u8off ? 0 : -128, hi = m->u8off ? 255 : 127;
+            const int64_t maxv = std::max<int64_t>({(int64_t)pk->merged_max, -(int64_t)pk->merged_min, 1});
+            if (pk->merged_min < lo || pk->merged_max > hi)
+                m->dense_why = "a row's values in one column sum past one byte (a row names a column twice)";
             else if (maxv * 128 * (int64_t)m->cols >= (int64_t(1) << 25))
                 m->dense_why = "scores may reach 2^25 (too many columns for the value width)";
             else if ((size_t)m->nkb * 1024 + 128 > kDenseLds)

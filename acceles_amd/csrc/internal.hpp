@@ -32,6 +32,7 @@ struct acc_packed {
     float scale = 0.f;
     int64_t nnz = 0, nnz_kept = 0;
     int64_t dup_rows = 0;                   // rows holding a column twice among kept entries
+    int32_t merged_max = 0, merged_min = 0; // range of the per (row, column) sums of kept values
     std::vector<int32_t> part_row_begin;    // split + 1
     std::vector<int64_t> part_slice_begin;  // split + 1
     std::vector<uint32_t> slice_off;        // in units of 64 words

@@ -107,7 +107,11 @@ __global__ __launch_bounds__(64) void dense_build(DenseArgs a) {
                 int sv = (int)v;
                 if (a.is_signed && (v >> (a.val_bits - 1))) sv = (int)(v | ~vmask);
                 const int kb = (int)(c >> 5), h = (int)((c >> 4) & 1u), bb = (int)(c & 15u);
-                l1[(kb * 64 + i + 32 * h) * 16 + bb] = (int8_t)(a.u8off ? sv - 128 : sv);
+                // added, not stored: a row that names a column twice keeps the
+                // sum (the host checked that every sum fits the byte; the
+                // cell starts at 0, or at -128 for the offset unsigned bytes)
+                int8_t &cell = l1[(kb * 64 + i + 32 * h) * 16 + bb];
+                cell = (int8_t)(cell + sv);
             }
         }
         wave_sync();

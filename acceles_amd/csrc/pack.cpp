@@ -68,12 +68,17 @@ extern "C" int acc_pack_lowbit(const acc_csr *m, const acc_pack_opts *o, acc_pac
     }
     pk->nnz_kept = nkept;
 
-    // rows that name a column twice (the reference generators sample columns
-    // with replacement, create_matrices.py:52-54); a dense layout cannot hold them
+    // Per (row, column) sums of the kept values: rows may name a column twice
+    // (the reference generators sample columns with replacement,
+    // create_matrices.py:52-54), and a dense layout holds a row's values in
+    // one column as their sum.  Records the rows with repeats and the range
+    // of the sums.
     int64_t dups = 0;
-#pragma omp parallel reduction(+ : dups)
+    int32_t mmax = 0, mmin = 0;
+#pragma omp parallel reduction(+ : dups) reduction(max : mmax) reduction(min : mmin)
     {
         std::vector<uint32_t> seen((size_t)(m->cols + 31) / 32 + 1, 0u);
+        std::vector<int32_t> sum((size_t)m->cols, 0);
 #pragma omp for schedule(static, 4096)
         for (int32_t r = 0; r < R; ++r) {
             bool dup = false;
@@ -84,12 +89,23 @@ extern "C" int acc_pack_lowbit(const acc_csr *m, const acc_pack_opts *o, acc_pac
                 uint32_t &w = seen[c >> 5];
                 if (w & (1u << (c & 31))) dup = true;
                 w |= 1u << (c & 31);
+                sum[c] += q[(size_t)j];
             }
-            for (int64_t j = b; j < e; ++j) seen[(uint32_t)m->idx[(size_t)j] >> 5] = 0u;
+            for (int64_t j = b; j < e; ++j) {
+                const uint32_t c = (uint32_t)m->idx[(size_t)j];
+                if (seen[c >> 5] & (1u << (c & 31))) {
+                    mmax = std::max(mmax, sum[c]);
+                    mmin = std::min(mmin, sum[c]);
+                    sum[c] = 0;
+                    seen[c >> 5] &= ~(1u << (c & 31));
+                }
+            }
             dups += dup;
         }
     }
     pk->dup_rows = dups;
+    pk->merged_max = mmax;
+    pk->merged_min = mmin;
 
     // partitions (quantify_util.py:185-189): rows/split each, remainder to the last
     const int S = o->split;

@@ -279,9 +279,10 @@ def test_lowbit_batch_tiny_partitions(acc, gpu, mode):
 
 
 def test_dense_layout_refuses_duplicate_columns(acc, gpu):
-    """A row naming a column twice cannot be one byte per column: the dense
-    mode is refused with a message, AUTO keeps the packet path, and the
-    results still match the oracle (which sums both entries)."""
+    """A row naming a column twice is one byte per column only while the two
+    values' sum fits: here it does not (255 + 255), so the dense mode is
+    refused with a message, AUTO keeps the packet path, and the results still
+    match the oracle (which sums both entries)."""
     rng = np.random.default_rng(9)
     rows, cols = 3000, 128
     ptr = np.arange(0, rows * 12 + 1, 12, dtype=np.int64)
@@ -292,12 +293,38 @@ def test_dense_layout_refuses_duplicate_columns(acc, gpu):
     csr = acc.CSRMatrix.from_arrays(rows, cols, ptr, idx, val)
     g = acc.GpuMatrix(csr, csr.pack(bits=8, split=8))
     assert not g.info.dense_ok
-    with pytest.raises(acc.AccelesError, match="twice"):
+    with pytest.raises(acc.AccelesError, match="sum past one byte"):
         g.set_batch_mode(acc.BATCH_DENSE)
     g.set_batch_mode(acc.BATCH_AUTO)
     X = acc.generate_queries(24, cols, seed=9)
     ks = np.array([KS[i % len(KS)] for i in range(24)], np.int32)
     check_batch(acc, g, csr, 8, False, X, ks, 256, 8)
+
+
+@pytest.mark.parametrize("bits,signed", [(4, False), (8, False), (6, True)])
+def test_dense_layout_sums_repeated_columns(acc, gpu, bits, signed):
+    """Columns sampled with replacement (as the reference generators do): a
+    row's values in one column are summed into its tile byte when the sums
+    fit, and the dense results equal the oracle's (which sums the entries)."""
+    rng = np.random.default_rng(bits)
+    rows, cols, d = 4000, 96, 16
+    ptr = np.arange(0, rows * d + 1, d, dtype=np.int64)
+    idx = np.sort(rng.integers(0, cols, (rows, d)), axis=1).astype(np.int32)
+    idx[0] = np.arange(d) * (cols // d)  # row 0: distinct columns, it holds the largest value
+    idx = idx.ravel()
+    val = rng.standard_normal(rows * d).astype(np.float32)
+    if not signed:
+        val = np.abs(val)
+    val[0] = 8.0 if not signed else -8.0  # the scale; the others stay near half of it or below
+    csr = acc.CSRMatrix.from_arrays(rows, cols, ptr, idx, val)
+    pk = csr.pack(bits=bits, split=8, is_signed=signed)
+    assert pk.info.nnz_kept > 0
+    g = acc.GpuMatrix(csr, pk)
+    assert g.info.dense_ok, "repeated columns whose sums fit a byte keep the dense layout"
+    g.set_batch_mode(acc.BATCH_DENSE)
+    X = acc.generate_queries(40, cols, seed=bits)
+    ks = np.array([KS[i % len(KS)] for i in range(40)], np.int32)
+    check_batch(acc, g, csr, bits, signed, X, ks, 256, 8)
 
 
 def test_fp64_exact_scores(acc, gpu):
