@@ -13,17 +13,20 @@
 // ---------------------------------------------------------------------------
 
 // y = A x with G lanes per row.  Lane g of a row sums its entries
-// g, g+G, g+2G, ... in that order with fmaf (the loads of up to U of them are
-// issued together), then an xor butterfly of plain adds.  Each wavefront owns
-// a contiguous range of rows.
-template <int G>
+// g, g+G, g+2G, ... in that order with fmaf, then an xor butterfly of plain
+// adds.  Each wavefront owns a contiguous range of rows and takes B blocks of
+// 64/G rows per step: the first U entries per lane of all B blocks are loaded
+// together (rows longer than U*G entries continue block by block), so a step
+// keeps B times more loads in flight than one block at a time.  The next
+// step's row bounds are fetched early.  x is staged in LDS.
+template <int G, int B>
 __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const int32_t *idx, const float *val,
                                                    const float *x, int rows, int cols, float *y) {
     extern __shared__ __align__(16) unsigned char smem[];
     float *xs = (float *)smem;
     for (int c = threadIdx.x; c < cols; c += ACC_WG) xs[c] = x[c];
     __syncthreads();
-    constexpr int RPI = 64 / G;  // rows per wavefront iteration
+    constexpr int RPI = 64 / G;  // rows per block
     constexpr int U = 4;
     const int lane = lane_id(), g = lane & (G - 1);
     const int nblocks = (rows + RPI - 1) / RPI;
@@ -31,35 +34,66 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
     const int per = (nblocks + nw - 1) / nw;
     const int b0 = gw * per, b1 = min(nblocks, b0 + per);
     if (b0 >= b1) return;
-    int r = b0 * RPI + lane / G;
-    uint32_t jb = r < rows ? ptr[r] : 0u, je = r < rows ? ptr[r + 1] : 0u;
-    for (int b = b0; b < b1; ++b) {
-        const int rn = r + RPI;
-        uint32_t nb = 0u, ne = 0u;
-        if (b + 1 < b1 && rn < rows) {  // next rows' bounds, fetched early
-            nb = ptr[rn];
-            ne = ptr[rn + 1];
+    uint32_t jb[B], je[B];
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+        const int r = (b0 + t) * RPI + lane / G;
+        const bool ok = b0 + t < b1 && r < rows;
+        jb[t] = ok ? ptr[r] : 0u;
+        je[t] = ok ? ptr[r + 1] : 0u;
+    }
+    for (int b = b0; b < b1; b += B) {
+        uint32_t nb[B], ne[B];
+#pragma unroll
+        for (int t = 0; t < B; ++t) {  // the next step's row bounds
+            const int r = (b + B + t) * RPI + lane / G;
+            const bool ok = b + B + t < b1 && r < rows;
+            nb[t] = ok ? ptr[r] : 0u;
+            ne[t] = ok ? ptr[r + 1] : 0u;
         }
-        float p = 0.f;
-        for (uint32_t j = jb + g; j < je; j += U * G) {
-            float v[U];
-            int c[U];
+        float v[B][U];
+        int c[B][U];
+#pragma unroll
+        for (int t = 0; t < B; ++t)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t jj = j + u * G;
-                v[u] = jj < je ? __builtin_nontemporal_load(val + jj) : 0.f;
-                c[u] = jj < je ? __builtin_nontemporal_load(idx + jj) : 0;
+                const uint32_t jj = jb[t] + g + u * G;
+                v[t][u] = jj < je[t] ? __builtin_nontemporal_load(val + jj) : 0.f;
+                c[t][u] = jj < je[t] ? __builtin_nontemporal_load(idx + jj) : 0;
             }
+        float p[B];
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            p[t] = 0.f;
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if (j + u * G < je) p = __builtin_fmaf(v[u], xs[c[u]], p);
+                if (jb[t] + g + u * G < je[t]) p[t] = __builtin_fmaf(v[t][u], xs[c[t][u]], p[t]);
+            for (uint32_t j = jb[t] + g + U * G; j < je[t]; j += U * G) {  // longer rows
+                float w[U];
+                int d[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t jj = j + u * G;
+                    w[u] = jj < je[t] ? __builtin_nontemporal_load(val + jj) : 0.f;
+                    d[u] = jj < je[t] ? __builtin_nontemporal_load(idx + jj) : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (j + u * G < je[t]) p[t] = __builtin_fmaf(w[u], xs[d[u]], p[t]);
+            }
         }
 #pragma unroll
-        for (int s = G / 2; s >= 1; s >>= 1) p = p + __shfl_xor(p, s);
-        if (g == 0 && r < rows) y[r] = p;
-        r = rn;
-        jb = nb;
-        je = ne;
+        for (int t = 0; t < B; ++t) {
+#pragma unroll
+            for (int s = G / 2; s >= 1; s >>= 1) p[t] = p[t] + __shfl_xor(p[t], s);
+            const int r = (b + t) * RPI + lane / G;
+            if (g == 0 && b + t < b1 && r < rows) y[r] = p[t];
+        }
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            jb[t] = nb[t];
+            je[t] = ne[t];
+        }
     }
 }
 
@@ -204,11 +238,11 @@ __global__ __launch_bounds__(ACC_WG) void spmv_fp64(const uint32_t *ptr, const i
 
 spmv_fn get_fp32_spmv(int G) {
     switch (G) {
-        case 2: return fp32_spmv<2>;
-        case 4: return fp32_spmv<4>;
-        case 8: return fp32_spmv<8>;
-        case 16: return fp32_spmv<16>;
-        case 32: return fp32_spmv<32>;
+        case 2: return fp32_spmv<2, 4>;
+        case 4: return fp32_spmv<4, 4>;
+        case 8: return fp32_spmv<8, 4>;
+        case 16: return fp32_spmv<16, 2>;
+        case 32: return fp32_spmv<32, 2>;
         default: return nullptr;
     }
 }
