@@ -470,7 +470,10 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
 
     LAUNCH(m, 0, get_dense_prep(), dim3(nqb * QB), dim3(ACC_WG), 0, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
-    dense_fn fs = get_dense_scores(NT, m->dense_bytes > kMallBytes);
+    // non-temporal tile loads only for one query block past the MALL size:
+    // with several blocks, the sibling blocks of a slice range run on one XCD
+    // and find the tiles in its L2
+    dense_fn fs = get_dense_scores(NT, m->dense_bytes > kMallBytes && nqb == 1);
     const size_t lds_s = (size_t)m->nkb * NT * 1024 + (size_t)QB * 4;
     if ((rc = lds_limit((const void *)fs, lds_s)) != ACC_OK) return rc;
     const int64_t groups = (m->nslices + 7) / 8;

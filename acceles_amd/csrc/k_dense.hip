@@ -448,20 +448,32 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
         offer_lists<4>(a, cb, k4, nv, s0, 0, ts, q, qx, have_qx);
     } else {
         constexpr int U = 4;
-        WaveTopM<2> ca;
-        ca.init(bufA, M);
-        for (int b = 0; b < nsl; b += 64 * U) {
-            uint32_t k[U];
+        int32_t ts = INT32_MIN;
+        if (nsl <= 1024 && M <= 32) {  // the M-th largest maximum by extraction, maxima in registers
+            uint32_t v[16];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = b + u * 64 + lane;
-                k[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
+            for (int u = 0; u < 16; ++u) {
+                const int i = u * 64 + lane;
+                v[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
             }
+            const uint32_t t = wave_kth_u32<16>(v, M);
+            if (t != 0u) ts = tk_score(t);
+        } else {
+            WaveTopM<2> ca;
+            ca.init(bufA, M);
+            for (int b = 0; b < nsl; b += 64 * U) {
+                uint32_t k[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) ca.push(k[u] != 0u, ((uint64_t)k[u] << 32) | (uint32_t)(b + u * 64 + lane));
+                for (int u = 0; u < U; ++u) {
+                    const int i = b + u * 64 + lane;
+                    k[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) ca.push(k[u] != 0u, ((uint64_t)k[u] << 32) | (uint32_t)(b + u * 64 + lane));
+            }
+            ca.compact();
+            if (ca.n >= M) ts = tk_score((uint32_t)(ca.buf[M - 1] >> 32));
         }
-        ca.compact();
-        const int32_t ts = ca.n >= M ? tk_score((uint32_t)(ca.buf[M - 1] >> 32)) : INT32_MIN;
         for (int b = 0; b < nsl; b += 64 * U) {
             uint4 k4[U];
             int nv[U];

@@ -245,6 +245,8 @@ def test_lowbit_batch_uneven_k(acc, gpu, mode):
     (20000, 500, (40, 100), 4, 32, False, 9),    # 4-bit, one 32-query block
     (30000, 1024, (0.5, 40.0), 8, 64, False, 33),  # power-law rows, 1024 columns
     (12000, 300, (5, 30), 4, 8, False, 300),     # 300 queries: three 128-query blocks, a ragged last one
+    (50000, 200, (10, 30), 8, 2, False, 20),     # partitions of 391 slices (threshold by extraction)
+    (150000, 200, (10, 30), 8, 2, False, 20),    # partitions of 1172 slices (threshold by collectors)
 ])
 def test_lowbit_batch_matches_oracle(acc, gpu, mode, rows, cols, deg, bits, split, signed, nq):
     kind = 1 if isinstance(deg[0], float) else 0
