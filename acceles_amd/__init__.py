@@ -112,6 +112,7 @@ SIGNATURES = [
     ("acc_csr_destroy", None, [_P]),
     ("acc_quantize", C.c_int, [C.c_int64, _F32P, C.c_int, C.c_int, C.POINTER(C.c_float), _I32P]),
     ("acc_permute_rows", C.c_int, [_P, C.c_uint64, C.POINTER(_P), _I32P]),
+    ("acc_resparse_csr", C.c_int, [_P, C.c_double, C.c_int, C.POINTER(_P)]),
     ("acc_ultracsr_entries", C.c_int, [C.c_int, C.c_int]),
     ("acc_pack_ultracsr", C.c_int64, [C.c_int64, _I32P, _I32P, _I32P, C.c_int32, C.c_int, C.c_int, _U64P, C.c_int64]),
     ("acc_unpack_ultracsr", C.c_int, [_U64P, C.c_int64, C.c_int, C.c_int, C.c_int32, _I32P, _I32P, _I32P]),
@@ -277,6 +278,13 @@ class CSRMatrix:
         perm = np.empty(self.rows, np.int32)
         _check(lib().acc_permute_rows(self._h, seed, C.byref(h), perm))
         return CSRMatrix(h), perm
+
+    def resparse(self, keep, mean_over_cells=False):
+        """ReSparse (resparse_csr, csr_util.py:152-187; resparse_spm with mean_over_cells): rows whose every |v|
+        exceeds the mean |v| stay whole, the others keep their ceil(n*keep) largest |v|."""
+        h = _P()
+        _check(lib().acc_resparse_csr(self._h, float(keep), int(mean_over_cells), C.byref(h)))
+        return CSRMatrix(h)
 
     def pack(self, bits=8, split=32, is_signed=False, col_bits=0):
         return PackedMatrix(self, bits=bits, split=split, is_signed=is_signed, col_bits=col_bits)

@@ -580,3 +580,64 @@ extern "C" int acc_permute_rows(const acc_csr *m, uint64_t seed, acc_csr **out, 
     *out = o;
     return ACC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// ReSparse: per-row pruning of small values (csr_util.py:152-187 over the
+// nonzeros; quantify_util.py:375-403 over all rows x cols cells)
+// ---------------------------------------------------------------------------
+
+extern "C" int acc_resparse_csr(const acc_csr *m, double keep, int mean_over_cells, acc_csr **out) {
+    if (!m || !out) return fail(ACC_ERR_ARG, "null argument");
+    if (!(keep >= 0.0 && keep <= 1.0)) return fail(ACC_ERR_ARG, "keep must be in [0, 1]");
+    // the mean importance |v| (FP64 sum)
+    double sum = 0.0;
+    for (float v : m->val) sum += std::fabs((double)v);
+    const double denom = mean_over_cells ? (double)m->rows * (double)m->cols : (double)m->val.size();
+    const double avg = denom > 0 ? sum / denom : 0.0;
+    std::vector<int32_t> kept((size_t)m->rows, 0);
+    std::vector<uint8_t> mask(m->val.size(), 0);
+#pragma omp parallel
+    {
+        std::vector<int64_t> order;
+#pragma omp for schedule(dynamic, 1024)
+        for (int32_t r = 0; r < m->rows; ++r) {
+            const int64_t b = m->ptr[(size_t)r], e = m->ptr[(size_t)r + 1], n = e - b;
+            if (n == 0) continue;
+            double mn = INFINITY;
+            for (int64_t j = b; j < e; ++j) mn = std::min(mn, std::fabs((double)m->val[(size_t)j]));
+            if (mn > avg) {  // every entry matters more than the mean: keep the row
+                std::fill(mask.begin() + b, mask.begin() + e, (uint8_t)1);
+                kept[(size_t)r] = (int32_t)n;
+                continue;
+            }
+            // keep ceil(n * keep) entries of largest |v| (equal |v|: the earlier entry)
+            const int64_t nk = (int64_t)std::ceil((double)n * keep);
+            order.resize((size_t)n);
+            std::iota(order.begin(), order.end(), b);
+            std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+                return std::fabs(m->val[(size_t)x]) > std::fabs(m->val[(size_t)y]);
+            });
+            for (int64_t t = 0; t < nk; ++t) mask[(size_t)order[(size_t)t]] = 1;
+            kept[(size_t)r] = (int32_t)nk;
+        }
+    }
+    acc_csr *o = new acc_csr();
+    o->rows = m->rows;
+    o->cols = m->cols;
+    o->ptr.assign((size_t)m->rows + 1, 0);
+    for (int32_t r = 0; r < m->rows; ++r) o->ptr[(size_t)r + 1] = o->ptr[(size_t)r] + kept[(size_t)r];
+    o->idx.resize((size_t)o->ptr[(size_t)m->rows]);
+    o->val.resize(o->idx.size());
+#pragma omp parallel for schedule(static, 4096)
+    for (int32_t r = 0; r < m->rows; ++r) {
+        int64_t d = o->ptr[(size_t)r];
+        for (int64_t j = m->ptr[(size_t)r]; j < m->ptr[(size_t)r + 1]; ++j)
+            if (mask[(size_t)j]) {  // kept entries stay in their column order
+                o->idx[(size_t)d] = m->idx[(size_t)j];
+                o->val[(size_t)d] = m->val[(size_t)j];
+                ++d;
+            }
+    }
+    *out = o;
+    return ACC_OK;
+}

@@ -54,12 +54,13 @@ def parse():
                     help="batches (embedding) or queries (large-single) in flight on as many streams")
     ap.add_argument("--batch-mode", choices=["auto", "packets", "dense"], default="auto",
                     help="scoring layout of the batched path")
-    ap.add_argument("--workload", choices=["embedding", "large-single", "batched", "serve", "copy"],
+    ap.add_argument("--workload", choices=["embedding", "large-single", "batched", "serve", "copy", "resparse"],
                     default="embedding",
                     help="embedding: 400,000x500 batched (default line); large-single: 20,000,000x1024 single "
                          "queries; batched: 2,000,000x900, 256 queries of mixed K per graph launch; serve: three "
                          "resident matrices under Poisson arrivals; copy: 1 GiB device-to-device copies, the "
-                         "calibration run for the HBM traffic counters")
+                         "calibration run for the HBM traffic counters; resparse: the ReSparse keep-ratio search on "
+                         "the embedding matrix, then single queries on the pruned packets")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="serve: seconds per offered rate")
     ap.add_argument("--deg-dist", choices=["uniform", "gamma"], default="uniform",
                     help="large-single: row lengths U[10,30] or Gamma(0.5, 40) (mean 20)")
@@ -224,25 +225,43 @@ def bench_batched(a, acc, stream):
     Xd = acc.DeviceBuffer.from_numpy(X)
     out = acc.DeviceBuffer((B, 100, 2), np.int32)
     plan = acc.Plan(g, acc.PLAN_LOWBIT_BATCH, Xd, ks, CK, out, 100)
-    dt = timed(stream, lambda: plan.launch(stream), a.steps, a.warmup)
-    qps = a.steps * B / dt
+    dt1 = timed(stream, lambda: plan.launch(stream), a.steps, a.warmup)
     res = out.numpy()
+    # batches in flight: one plan (own workspace) per stream
+    NF = a.inflight
+    streams = [stream] + [acc.Stream(0) for _ in range(1, NF)]
+    xs2 = [Xd] + [acc.DeviceBuffer.from_numpy(acc.generate_queries(B, cols, seed=0xACCE1E5 + 60 + j))
+                  for j in range(1, NF)]
+    outs2 = [out] + [acc.DeviceBuffer((B, 100, 2), np.int32) for _ in range(1, NF)]
+    plans = [plan] + [acc.Plan(g, acc.PLAN_LOWBIT_BATCH, xs2[j], ks, CK, outs2[j], 100) for j in range(1, NF)]
+    it = [0]
+
+    def pipe_step():
+        j = it[0] % NF
+        plans[j].launch(streams[j])
+        it[0] += 1
+    dt = timed_multi(streams, pipe_step, a.steps, a.warmup)
+    qps = a.steps * B / dt
+    same = bool(np.array_equal(out.numpy(), res))
     prec = []
     for j in range(0, B, 32):
         k = int(ks[j])
         ex = exact_topk(g, acc, Xd.at(j * cols), rows, 100, stream)
         prec.append(len(set(res[j, :k, 1]) & set(ex[:k])) / k)
-    log(f"[bench] batched: {qps:.0f} q/s ({dt * 1e3 / a.steps:.3f} ms per {B}-query graph launch); "
-        f"mean precision@K {np.mean(prec):.4f}")
+    log(f"[bench] batched: {qps:.0f} q/s with {NF} launches in flight ({dt * 1e3 / a.steps:.3f} ms per {B}-query "
+        f"graph launch; {a.steps * B / dt1:.0f} q/s one at a time); mean precision@K {np.mean(prec):.4f}")
     line = line_common(a, "batched low-bit Top-K queries/s (mixed K)", qps, "queries/s", dt * 1e3 / a.steps,
                        "u8" if a.bits == 8 else f"u{a.bits}",
                        f"low-bit two-stage, batched: {rows}x{cols}, row nnz U[30,90], L2-normalised, {a.bits}-bit, "
-                       f"SPLIT=32, CK=512, {B} queries per HIP-graph launch, K from {{1,8,16,32,50,75,100}}",
+                       f"SPLIT=32, CK=512, {B} queries per HIP-graph launch, K from {{1,8,16,32,50,75,100}}, "
+                       f"{NF} launches in flight (a plan per stream)",
                        {"rows": rows, "cols": cols, "nnz": csr.nnz, "queries_per_step": B})
     line.update({"roofline": None, "cpu_baseline": None, "mean_precision_at_k": round(float(np.mean(prec)), 5),
+                 "one_at_a_time_qps": round(a.steps * B / dt1, 2), "in_flight_equals_one_at_a_time": same,
                  "autotune": [{"nq": p.nq, "layout": int(p.layout), "us_single": round(p.us_single, 1),
                                "us_packets": round(p.us_sparse, 1), "us_dense": round(p.us_dense, 1)} for p in pts]})
-    plan.close()
+    for p_ in plans:
+        p_.close()
     print(json.dumps(line), flush=True)
 
 
@@ -293,6 +312,57 @@ def bench_copy(a, acc, stream):
           flush=True)
 
 
+def bench_resparse(a, acc, stream):
+    """The embedding workload's matrix: ReSparse keep ratio searched at 0.999 of the unpruned convergence metric
+    (100 queries, its SPLIT and CK), then single queries on the pruned packets, rescored on the unpruned FP32 rows."""
+    from acceles_amd.resparse import search_keep_ratio
+    K, CK = a.k, a.ck
+    csr = acc.CSRMatrix.generate(a.rows, a.cols, deg=tuple(a.deg), seed=0xACCE1E5 + 1)
+    t0 = time.perf_counter()
+    res = search_keep_ratio(csr, bits=a.bits, split=a.split, ck=CK, nq=100, accuracy_threshold=0.999, log=log)
+    t_search = time.perf_counter() - t0
+    log(f"[bench] resparse: keep {res['keep']:.3f} after {len(res['history'])} epochs ({t_search:.1f}s)")
+    nq = 64
+    X = acc.generate_queries(nq, a.cols, seed=0xACCE1E5 + 9)
+    Xd = acc.DeviceBuffer.from_numpy(X)
+    xs = [Xd.at(i * a.cols) for i in range(nq)]
+    out = acc.DeviceBuffer((nq, K, 2), np.int32)
+    os_ = [out.at(i * K * 2) for i in range(nq)]
+    g_exact = acc.GpuMatrix(csr, None)
+    exact = [exact_topk(g_exact, acc, xs[i], a.rows, K, stream) for i in range(nq)]
+    del g_exact
+    runs = {}
+    for name, m in (("unpruned", csr), ("pruned", csr.resparse(res["keep"]))):
+        pk = m.pack(bits=a.bits, split=a.split)
+        g = acc.GpuMatrix(csr, pk)  # pruned packets for stage 1, unpruned FP32 rows for the rescoring
+        i = [0]
+
+        def step():
+            g.topk_lowbit(xs[i[0] % nq], K, CK, os_[i[0] % nq], stream)
+            i[0] += 1
+        dt = timed(stream, step, a.steps, a.warmup)
+        for j in range(nq):
+            g.topk_lowbit(xs[j], K, CK, os_[j], stream)
+        stream.synchronize()
+        res_ = out.numpy()
+        prec = np.mean([precision_at(res_[j, :, 1], exact[j]) for j in range(nq)], axis=0)
+        s1 = g.time_lowbit_stage(xs[0], K, CK, 0, 40, stream)
+        runs[name] = {"nnz": m.nnz, "stream_bytes": pk.info.stream_bytes, "qps": round(a.steps / dt, 2),
+                      "slicemax_us": round(s1 * 1e3, 2),
+                      "precision_at_k": dict(zip(map(str, KS), np.round(prec, 5).tolist()))}
+        log(f"[bench] {name}: nnz {m.nnz}, packets {pk.info.stream_bytes / 1e6:.1f} MB, {a.steps / dt:.0f} q/s, "
+            f"slicemax {s1 * 1e3:.1f}us, precision@K {runs[name]['precision_at_k']}")
+        del g
+    line = line_common(a, "single-query low-bit Top-100 queries/s after ReSparse", runs["pruned"]["qps"],
+                       "queries/s", 1e3 / runs["pruned"]["qps"], "u8",
+                       f"ReSparse keep ratio searched at 0.999 convergence (100 queries), then single queries: "
+                       f"{a.rows}x{a.cols}, row nnz U[{a.deg[0]},{a.deg[1]}], {a.bits}-bit, SPLIT={a.split}, CK={CK}",
+                       {"rows": a.rows, "cols": a.cols})
+    line.update({"roofline": None, "cpu_baseline": None, "keep": res["keep"], "search_seconds": round(t_search, 2),
+                 "history": [(round(k, 3), round(c, 5), n) for k, c, n in res["history"]], "runs": runs})
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -303,7 +373,7 @@ def main():
     if a.workload != "embedding":
         stream = acc.Stream(0)
         {"large-single": bench_large_single, "batched": bench_batched, "serve": bench_serve,
-         "copy": bench_copy}[a.workload](a, acc, stream)
+         "copy": bench_copy, "resparse": bench_resparse}[a.workload](a, acc, stream)
         return
 
     assert acc.device_count() > 0, "no GPU visible"

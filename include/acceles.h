@@ -110,6 +110,15 @@ int acc_quantize(int64_t n, const float *v, int bits, int symmetric, float *scal
  * csr_util.py:101-114), seeded: out row i is input row perm[i]. */
 int acc_permute_rows(const acc_csr *csr, uint64_t seed, acc_csr **out, int32_t *perm);
 
+/* ReSparse (resparse_csr, csr_util.py:152-187; resparse_spm,
+ * quantify_util.py:375-403): with avg = the mean |v| over the nonzeros
+ * (mean_over_cells = 0) or over all rows*cols cells (1, the dense form), a
+ * row whose smallest |v| exceeds avg is kept whole; any other row keeps its
+ * ceil(n * keep) entries of largest |v| (equal |v|: the earlier entry; the
+ * reference's unstable argsort leaves that order open), in column order.
+ * The mean is summed in FP64.  keep in [0, 1]. */
+int acc_resparse_csr(const acc_csr *csr, double keep, int mean_over_cells, acc_csr **out);
+
 /* ---- Ultra-CSR reference packet format ---------------------------- */
 
 /* Entries per 512-bit packet: 511 / (1 + col_bits + val_bits)

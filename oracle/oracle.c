@@ -391,3 +391,32 @@ void orc_top_k_spmv_f32(int32_t nrows, const int64_t *ptr, const int32_t *idx,
     }
     free(h);
 }
+
+/* ------------------------------------------------------------------ */
+/* ReSparse (csr_util.py:152-187; quantify_util.py:375-403)            */
+/* ------------------------------------------------------------------ */
+
+void orc_resparse_mask(int32_t rows, int32_t cols, const int64_t *ptr, const float *val, double keep,
+                       int mean_over_cells, uint8_t *mask) {
+    double sum = 0.0;
+    const int64_t nnz = ptr[rows];
+    for (int64_t j = 0; j < nnz; ++j) sum += fabs((double)val[j]);
+    const double denom = mean_over_cells ? (double)rows * (double)cols : (double)nnz;
+    const double avg = denom > 0 ? sum / denom : 0.0;
+    for (int32_t r = 0; r < rows; ++r) {
+        const int64_t b = ptr[r], e = ptr[r + 1], n = e - b;
+        int all = n > 0;
+        for (int64_t j = b; j < e; ++j)
+            if (!(fabs((double)val[j]) > avg)) all = 0;
+        const int64_t nk = all ? n : (int64_t)ceil((double)n * keep);
+        /* entry j's place in the row's order by |v| descending, earlier first */
+        for (int64_t j = b; j < e; ++j) {
+            int64_t rank = 0;
+            for (int64_t t = b; t < e; ++t) {
+                const double a = fabs((double)val[t]), c = fabs((double)val[j]);
+                if (a > c || (a == c && t < j)) ++rank;
+            }
+            mask[j] = rank < nk;
+        }
+    }
+}

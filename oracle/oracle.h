@@ -34,6 +34,9 @@
  *                          quantify_util.py:185-189
  *   orc_rescore_topk       orc_spmv_f32_grouped + orc_topk on the candidate rows
  *   orc_precision          hand-computed examples of quantify_util.py:198-241
+ *   orc_resparse_mask      hand-computed rows (kept whole, ceil rounding,
+ *                          ties, empty rows, both means) and a sort-based
+ *                          numpy restatement on random rows
  *   orc_ultracsr_*         x-decoder known-answer test (test_x_decoder.cpp:6),
  *                          bit positions of fpga_utils.hpp:438-503, round trip
  */
@@ -108,6 +111,14 @@ int orc_two_stage(int32_t nrows, int32_t ncols, const int64_t *ptr,
  * |set(cand) & set(exact[0:K])| / K. */
 void orc_precision(const int32_t *exact, int ncand, const int32_t *cand,
                    int nks, const int32_t *ks, double *prec_out);
+
+/* ReSparse keep mask (resparse_csr, csr_util.py:152-187; resparse_spm,
+ * quantify_util.py:375-403 with mean_over_cells): avg = mean |v| over the
+ * nonzeros (or all rows*cols cells); a row whose every |v| > avg is kept
+ * whole, any other keeps its ceil(n*keep) entries of largest |v| (equal |v|:
+ * the earlier entry).  By rank counting per entry (brute force). */
+void orc_resparse_mask(int32_t rows, int32_t cols, const int64_t *ptr, const float *val, double keep,
+                       int mean_over_cells, uint8_t *mask);
 
 /* x-decoder (fpga_utils.hpp:279-302): x[i] = carry + popcount(flags &
  * ((2 << i) - 1)) for i < n. */

@@ -24,6 +24,7 @@ _I64 = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _F32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _F64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _U64 = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_U8 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 
 _orc = None
 _ref = None
@@ -61,6 +62,8 @@ def oracle() -> C.CDLL:
         L.orc_ultracsr_unpack.restype = None
         L.orc_top_k_spmv_f32.argtypes = [C.c_int32, _I64, _I32, _F32, _F32, C.c_int, _F32, _I32, _F32]
         L.orc_top_k_spmv_f32.restype = None
+        L.orc_resparse_mask.argtypes = [C.c_int32, C.c_int32, _I64, _F32, C.c_double, C.c_int, _U8]
+        L.orc_resparse_mask.restype = None
         _orc = L
     return _orc
 
@@ -221,3 +224,11 @@ def top_k_spmv_f32(ptr, idx, val, x, k):
     ov = np.empty(k, np.float32)
     oracle().orc_top_k_spmv_f32(n, ptr, idx, val, x, k, y, oi, ov)
     return oi, ov
+
+
+def resparse_mask(ptr, val, cols, keep, mean_over_cells=False):
+    ptr = np.ascontiguousarray(ptr, np.int64)
+    mask = np.empty(int(ptr[-1]), np.uint8)
+    oracle().orc_resparse_mask(len(ptr) - 1, cols, ptr, np.ascontiguousarray(val, np.float32), float(keep),
+                               int(mean_over_cells), mask)
+    return mask.astype(bool)
