@@ -113,6 +113,11 @@ SIGNATURES = [
     ("acc_quantize", C.c_int, [C.c_int64, _F32P, C.c_int, C.c_int, C.POINTER(C.c_float), _I32P]),
     ("acc_permute_rows", C.c_int, [_P, C.c_uint64, C.POINTER(_P), _I32P]),
     ("acc_resparse_csr", C.c_int, [_P, C.c_double, C.c_int, C.POINTER(_P)]),
+    ("acc_ndcg_rank", C.c_int, [_I32P, _I32P, C.c_int, C.POINTER(C.c_double)]),
+    ("acc_ndcg_score", C.c_int, [_I32P, C.POINTER(C.c_double), C.c_int, _I32P, C.c_int, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("acc_edit_distance", C.c_int, [_I32P, C.c_int, _I32P, C.c_int, C.POINTER(C.c_uint32)]),
+    ("acc_count_errors", C.c_int, [_I32P, _I32P, C.c_int, C.POINTER(C.c_int32)]),
     ("acc_ultracsr_entries", C.c_int, [C.c_int, C.c_int]),
     ("acc_pack_ultracsr", C.c_int64, [C.c_int64, _I32P, _I32P, _I32P, C.c_int32, C.c_int, C.c_int, _U64P, C.c_int64]),
     ("acc_unpack_ultracsr", C.c_int, [_U64P, C.c_int64, C.c_int, C.c_int, C.c_int32, _I32P, _I32P, _I32P]),
@@ -293,6 +298,63 @@ class CSRMatrix:
         if getattr(self, "_h", None) and self._h.value and _lib is not None:
             _lib.acc_csr_destroy(self._h)
             self._h = _P()
+
+
+# ---- ranking-quality metrics (evaluation_utils.hpp:112-267, quantify_util.py:313-329) ----
+
+def ndcg_rank(golden, test):
+    """normalized_discounted_cumulative_gain (evaluation_utils.hpp:112-147) of two row lists of one length."""
+    g, t = _i32(golden), _i32(test)
+    if len(g) != len(t):
+        raise AccelesError(ACC_ERR_ARG, "ndcg_rank: lists of different lengths")
+    out = C.c_double()
+    _check(lib().acc_ndcg_rank(g, t, len(g), C.byref(out)))
+    return out.value
+
+
+def bounded_ndcg(golden, test, bounds=(10, 20, 50)):
+    """bounded_ndcg (evaluation_utils.hpp:152-183): ndcg_rank of the first b rows for each bound b <= len."""
+    out = []
+    for b in bounds:
+        if b > len(golden):
+            break
+        out.append(ndcg_rank(golden[:b], test[:b]))
+    return out
+
+
+def ndcg_score(sw_idx, sw_val, hw_idx):
+    """ndcg (quantify_util.py:313-329): returns (ndcg, dcg, idcg)."""
+    si, hi = _i32(sw_idx), _i32(hw_idx)
+    sv = np.ascontiguousarray(sw_val, np.float64)
+    n, d, i = C.c_double(), C.c_double(), C.c_double()
+    _check(lib().acc_ndcg_score(si, sv.ctypes.data_as(C.POINTER(C.c_double)), len(si), hi, len(hi), C.byref(n),
+                                C.byref(d), C.byref(i)))
+    return n.value, d.value, i.value
+
+
+def edit_distance(a, b):
+    """edit_distance (evaluation_utils.hpp:197-209)."""
+    x, y = _i32(a), _i32(b)
+    out = C.c_uint32()
+    _check(lib().acc_edit_distance(x, len(x), y, len(y), C.byref(out)))
+    return out.value
+
+
+def bounded_edit_distance(golden, test, bounds=(10, 20, 50)):
+    """bounded_edit_distance (evaluation_utils.hpp:211-237)."""
+    return [edit_distance(golden[:b], test[:b]) for b in bounds if b <= len(golden)]
+
+
+def bounded_count_errors(golden, test, bounds=(10, 20, 50)):
+    """bounded_count_errors (evaluation_utils.hpp:239-263)."""
+    out = []
+    for b in bounds:
+        if b > len(golden):
+            break
+        e = C.c_int32()
+        _check(lib().acc_count_errors(_i32(golden[:b]), _i32(test[:b]), b, C.byref(e)))
+        out.append(e.value)
+    return out
 
 
 def read_coo_matrix(path: str, zero_index: bool = True) -> COOMatrix:

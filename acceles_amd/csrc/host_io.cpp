@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <unordered_map>
 
 // ---------------------------------------------------------------------------
 // errors
@@ -639,5 +640,71 @@ extern "C" int acc_resparse_csr(const acc_csr *m, double keep, int mean_over_cel
             }
     }
     *out = o;
+    return ACC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ranking-quality metrics (evaluation_utils.hpp:112-267, quantify_util.py:313-329)
+// ---------------------------------------------------------------------------
+
+extern "C" int acc_ndcg_rank(const int32_t *golden, const int32_t *test, int n, double *out) {
+    if ((n > 0 && (!golden || !test)) || !out || n < 0) return fail(ACC_ERR_ARG, "bad arguments");
+    // relevance of a row: n - (its place in the list); for a row that appears
+    // twice in test, the later place counts
+    std::unordered_map<int32_t, int> rel;
+    for (int i = 0; i < n; ++i) rel[test[i]] = n - i;
+    double dcg = 0.0, idcg = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const int g = n - i;
+        const auto it = rel.find(golden[i]);
+        const int t = it == rel.end() ? 0 : it->second;
+        // the argument is a float (place + 2) and the logarithm a double one,
+        // as in the reference's expression
+        const double disc = std::log2((double)((float)std::abs(g - n) + 2));
+        dcg += (double)t / disc;
+        idcg += (double)g / disc;
+    }
+    *out = dcg / idcg;
+    return ACC_OK;
+}
+
+extern "C" int acc_ndcg_score(const int32_t *sw_idx, const double *sw_val, int nsw, const int32_t *hw_idx, int nhw,
+                              double *ndcg, double *dcg_out, double *idcg_out) {
+    if (nsw < 0 || nhw < 0 || (nsw > 0 && (!sw_idx || !sw_val)) || (nhw > 0 && !hw_idx))
+        return fail(ACC_ERR_ARG, "bad arguments");
+    std::unordered_map<int32_t, double> rel;  // a row listed twice: the later value counts
+    for (int i = 0; i < nsw; ++i) rel[sw_idx[i]] = sw_val[i];
+    double dcg = 0.0, idcg = 0.0;
+    for (int i = 0; i < nhw; ++i) {
+        const auto it = rel.find(hw_idx[i]);
+        dcg += (it == rel.end() ? 0.0 : it->second) / std::log2((double)(i + 2));
+    }
+    for (int i = 0; i < nsw; ++i) idcg += sw_val[i] / std::log2((double)(i + 2));
+    if (ndcg) *ndcg = dcg / idcg;
+    if (dcg_out) *dcg_out = dcg;
+    if (idcg_out) *idcg_out = idcg;
+    return ACC_OK;
+}
+
+extern "C" int acc_edit_distance(const int32_t *a, int na, const int32_t *b, int nb, uint32_t *out) {
+    if (na < 0 || nb < 0 || (na > 0 && !a) || (nb > 0 && !b) || !out) return fail(ACC_ERR_ARG, "bad arguments");
+    std::vector<uint32_t> prev((size_t)nb + 1), cur((size_t)nb + 1);  // one row of the Levenshtein table
+    for (int j = 0; j <= nb; ++j) prev[(size_t)j] = (uint32_t)j;
+    for (int i = 1; i <= na; ++i) {
+        cur[0] = (uint32_t)i;
+        for (int j = 1; j <= nb; ++j)
+            cur[(size_t)j] = std::min({prev[(size_t)j] + 1, cur[(size_t)j - 1] + 1,
+                                       prev[(size_t)j - 1] + (a[i - 1] == b[j - 1] ? 0u : 1u)});
+        std::swap(prev, cur);
+    }
+    *out = prev[(size_t)nb];
+    return ACC_OK;
+}
+
+extern "C" int acc_count_errors(const int32_t *golden, const int32_t *test, int n, int32_t *out) {
+    if (n < 0 || (n > 0 && (!golden || !test)) || !out) return fail(ACC_ERR_ARG, "bad arguments");
+    int32_t e = 0;
+    for (int i = 0; i < n; ++i) e += golden[i] != test[i];
+    *out = e;
     return ACC_OK;
 }

@@ -496,7 +496,7 @@ def main():
 
     # ---- accuracy vs FP64 exact Top-K ----
     y64 = acc.DeviceBuffer((a.rows,), np.float64)
-    prec, allfound = [], []
+    prec, allfound, ndcgs = [], [], []
     for i in range(B):
         g.spmv_fp64(xs[i], y64, stream)
         stream.synchronize()
@@ -507,9 +507,12 @@ def main():
         p = [len(set(got[:k]) & set(exact[:k])) / k for k in KS]
         prec.append(p)
         allfound.append([float(x == 1.0) for x in p])
+        # score NDCG (quantify_util.py:313-329): relevance = exact FP64 score
+        ndcgs.append([acc.ndcg_score(exact[:k], yy[exact[:k]], got[:k])[0] for k in KS])
     precision = dict(zip(map(str, KS), np.round(np.mean(prec, axis=0), 5).tolist()))
     recall_all = dict(zip(map(str, KS), np.round(np.mean(allfound, axis=0), 5).tolist()))
-    log(f"[bench] precision@K {precision}")
+    ndcg_at_k = dict(zip(map(str, KS), np.round(np.mean(ndcgs, axis=0), 6).tolist()))
+    log(f"[bench] precision@K {precision}; NDCG@K {ndcg_at_k}")
 
     # ---- HBM traffic from rocprof counters, when profiled for this workload ----
     traffic = pmc_traffic(sc_kernel, a.rows, a.cols, a.bits)
@@ -590,6 +593,7 @@ def main():
         "bytes_per_nnz_fp32": round(g.info.csr_bytes / csr.nnz, 4),
         "precision_at_k": precision,
         "recall_all_found_at_k": recall_all,
+        "ndcg_at_k": ndcg_at_k,
     }
     print(json.dumps(line), flush=True)
 

@@ -119,6 +119,25 @@ int acc_permute_rows(const acc_csr *csr, uint64_t seed, acc_csr **out, int32_t *
  * The mean is summed in FP64.  keep in [0, 1]. */
 int acc_resparse_csr(const acc_csr *csr, double keep, int mean_over_cells, acc_csr **out);
 
+/* ---- Ranking-quality metrics ---------------------------------------- */
+
+/* Rank NDCG of a test list against a golden list of n rows
+ * (normalized_discounted_cumulative_gain, evaluation_utils.hpp:112-147):
+ * relevance n - place (a row listed twice in test: the later place);
+ * discount log2(place + 2). */
+int acc_ndcg_rank(const int32_t *golden, const int32_t *test, int n, double *out);
+/* Score NDCG (ndcg, quantify_util.py:313-329): relevance of a row = its exact
+ * score in the sw list (0 when absent), dcg over the hw list's places,
+ * idcg over the sw list; any output pointer may be null. */
+int acc_ndcg_score(const int32_t *sw_idx, const double *sw_val, int nsw, const int32_t *hw_idx, int nhw,
+                   double *ndcg, double *dcg, double *idcg);
+/* Levenshtein distance of two row lists (edit_distance,
+ * evaluation_utils.hpp:197-209). */
+int acc_edit_distance(const int32_t *a, int na, const int32_t *b, int nb, uint32_t *out);
+/* Places where the lists differ (bounded_count_errors' inner count,
+ * evaluation_utils.hpp:241-252). */
+int acc_count_errors(const int32_t *golden, const int32_t *test, int n, int32_t *out);
+
 /* ---- Ultra-CSR reference packet format ---------------------------- */
 
 /* Entries per 512-bit packet: 511 / (1 + col_bits + val_bits)

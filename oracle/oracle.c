@@ -420,3 +420,19 @@ void orc_resparse_mask(int32_t rows, int32_t cols, const int64_t *ptr, const flo
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* score NDCG (quantify_util.py:313-329)                               */
+/* ------------------------------------------------------------------ */
+
+double orc_ndcg_score(const int32_t *sw_idx, const double *sw_val, int nsw, const int32_t *hw_idx, int nhw) {
+    double dcg = 0.0, idcg = 0.0;
+    for (int i = 0; i < nhw; ++i) {
+        double rel = 0.0; /* the last listing of the row in sw, as a dict built in order keeps */
+        for (int j = 0; j < nsw; ++j)
+            if (sw_idx[j] == hw_idx[i]) rel = sw_val[j];
+        dcg += rel / log2((double)i + 2.0);
+    }
+    for (int i = 0; i < nsw; ++i) idcg += sw_val[i] / log2((double)i + 2.0);
+    return dcg / idcg;
+}

@@ -34,6 +34,8 @@
  *                          quantify_util.py:185-189
  *   orc_rescore_topk       orc_spmv_f32_grouped + orc_topk on the candidate rows
  *   orc_precision          hand-computed examples of quantify_util.py:198-241
+ *   orc_ndcg_score         hand-computed lists (equal lists give 1, a
+ *                          swap, a miss, a repeated row)
  *   orc_resparse_mask      hand-computed rows (kept whole, ceil rounding,
  *                          ties, empty rows, both means) and a sort-based
  *                          numpy restatement on random rows
@@ -119,6 +121,10 @@ void orc_precision(const int32_t *exact, int ncand, const int32_t *cand,
  * the earlier entry).  By rank counting per entry (brute force). */
 void orc_resparse_mask(int32_t rows, int32_t cols, const int64_t *ptr, const float *val, double keep,
                        int mean_over_cells, uint8_t *mask);
+
+/* Score NDCG (quantify_util.py:313-329): relevance of a row = its sw score
+ * (the last listing; 0 when absent), dcg over hw places, idcg over sw. */
+double orc_ndcg_score(const int32_t *sw_idx, const double *sw_val, int nsw, const int32_t *hw_idx, int nhw);
 
 /* x-decoder (fpga_utils.hpp:279-302): x[i] = carry + popcount(flags &
  * ((2 << i) - 1)) for i < n. */

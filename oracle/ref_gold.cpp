@@ -12,6 +12,8 @@
 //   spmv_coo_gold_top_k        gold_algorithms.hpp:189-247
 //   spmv_coo_gold_top_k_2      gold_algorithms.hpp:249-308
 //   sort_tuples                src/common/utils/evaluation_utils.hpp:40-62
+//   normalized_discounted_cumulative_gain, edit_distance, bounded_count_errors
+//                              evaluation_utils.hpp:112-252
 #include "types.hpp"
 #include "csc_matrix/csc_matrix.hpp"
 #include "utils/mmio.hpp"
@@ -83,6 +85,20 @@ void ref_sort_tuples_i32(unsigned n, unsigned *idx, int *val) {
 
 void ref_sort_tuples_f32(unsigned n, unsigned *idx, float *val) {
     sort_tuples<unsigned, float>(n, idx, val);
+}
+
+
+double ref_ndcg(const int *golden, const int *test, int n) {
+    return normalized_discounted_cumulative_gain(std::vector<int>(golden, golden + n), std::vector<int>(test, test + n));
+}
+
+unsigned ref_edit_distance(const int *a, int na, const int *b, int nb) {
+    return edit_distance(std::vector<int>(a, a + na), std::vector<int>(b, b + nb));
+}
+
+int ref_count_errors(const int *golden, const int *test, int n) {
+    std::vector<int> bounds{n};
+    return bounded_count_errors(std::vector<int>(golden, golden + n), std::vector<int>(test, test + n), bounds)[0];
 }
 
 }  // extern "C"

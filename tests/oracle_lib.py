@@ -64,6 +64,8 @@ def oracle() -> C.CDLL:
         L.orc_top_k_spmv_f32.restype = None
         L.orc_resparse_mask.argtypes = [C.c_int32, C.c_int32, _I64, _F32, C.c_double, C.c_int, _U8]
         L.orc_resparse_mask.restype = None
+        L.orc_ndcg_score.argtypes = [_I32, _F64, C.c_int, _I32, C.c_int]
+        L.orc_ndcg_score.restype = C.c_double
         _orc = L
     return _orc
 
@@ -90,6 +92,12 @@ def ref() -> C.CDLL:
         L.ref_sort_tuples_i32.restype = None
         L.ref_sort_tuples_f32.argtypes = [C.c_uint, _U32, _F32]
         L.ref_sort_tuples_f32.restype = None
+        L.ref_ndcg.argtypes = [_I32, _I32, C.c_int]
+        L.ref_ndcg.restype = C.c_double
+        L.ref_edit_distance.argtypes = [_I32, C.c_int, _I32, C.c_int]
+        L.ref_edit_distance.restype = C.c_uint
+        L.ref_count_errors.argtypes = [_I32, _I32, C.c_int]
+        L.ref_count_errors.restype = C.c_int
         _ref = L
     return _ref
 
@@ -232,3 +240,9 @@ def resparse_mask(ptr, val, cols, keep, mean_over_cells=False):
     oracle().orc_resparse_mask(len(ptr) - 1, cols, ptr, np.ascontiguousarray(val, np.float32), float(keep),
                                int(mean_over_cells), mask)
     return mask.astype(bool)
+
+
+def ndcg_score(sw_idx, sw_val, hw_idx):
+    sw_idx = np.ascontiguousarray(sw_idx, np.int32)
+    hw_idx = np.ascontiguousarray(hw_idx, np.int32)
+    return oracle().orc_ndcg_score(sw_idx, np.ascontiguousarray(sw_val, np.float64), len(sw_idx), hw_idx, len(hw_idx))
