@@ -126,6 +126,10 @@ SIGNATURES = [
     ("acc_packed_info", C.c_int, [_P, C.POINTER(PackedInfo)]),
     ("acc_unpack_lowbit", C.c_int, [_P, _I64P, _P, _P]),
     ("acc_packed_destroy", None, [_P]),
+    ("acc_csr_save", C.c_int, [_P, C.c_char_p]),
+    ("acc_csr_load", C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    ("acc_packed_save", C.c_int, [_P, C.c_char_p]),
+    ("acc_packed_load", C.c_int, [C.c_char_p, C.POINTER(_P)]),
     ("acc_generate_csr", C.c_int, [C.c_int32, C.c_int32, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.c_uint64, C.POINTER(_P)]),
     ("acc_generate_queries", C.c_int, [C.c_int32, C.c_int32, C.c_int, C.c_int, C.c_uint64, _F32P]),
@@ -283,6 +287,17 @@ class CSRMatrix:
         perm = np.empty(self.rows, np.int32)
         _check(lib().acc_permute_rows(self._h, seed, C.byref(h), perm))
         return CSRMatrix(h), perm
+
+    def save(self, path):
+        """Binary CSR file (acc_csr_save)."""
+        _check(lib().acc_csr_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path):
+        """CSR matrix from a binary file (acc_csr_load)."""
+        h = _P()
+        _check(lib().acc_csr_load(os.fsencode(path), C.byref(h)))
+        return cls(h)
 
     def resparse(self, keep, mean_over_cells=False):
         """ReSparse (resparse_csr, csr_util.py:152-187; resparse_spm with mean_over_cells): rows whose every |v|
@@ -457,6 +472,21 @@ class PackedMatrix:
         self._h = h
         self.info = PackedInfo()
         _check(lib().acc_packed_info(self._h, C.byref(self.info)))
+
+    def save(self, path):
+        """Binary packet file (acc_packed_save)."""
+        _check(lib().acc_packed_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path):
+        """Packets from a binary file (acc_packed_load)."""
+        h = _P()
+        _check(lib().acc_packed_load(os.fsencode(path), C.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self.info = PackedInfo()
+        _check(lib().acc_packed_info(self._h, C.byref(self.info)))
+        return self
 
     def unpack(self):
         """(ptr, idx, qval) of the kept (non-zero quantized) entries, CSR order."""

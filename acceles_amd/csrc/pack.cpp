@@ -16,6 +16,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 
 using acc::fail;
@@ -318,5 +319,118 @@ extern "C" int acc_unpack_ultracsr(const uint64_t *packets, int64_t nnz, int col
             if (qval) qval[i] = (int32_t)v;
         }
     }
+    return ACC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Binary files of CSR matrices and packets (the load-time cache the reference
+// lacks: its drivers re-read .mtx text and re-pack on every run)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+const char kCsrMagic[8] = {'A', 'C', 'C', 'C', 'S', 'R', '0', '1'};
+const char kPackMagic[8] = {'A', 'C', 'C', 'P', 'K', 'T', '0', '1'};
+
+struct File {
+    FILE *f = nullptr;
+    explicit File(const char *path, const char *mode) : f(std::fopen(path, mode)) {}
+    ~File() {
+        if (f) std::fclose(f);
+    }
+};
+
+template <typename T>
+bool put(FILE *f, const T &v) { return std::fwrite(&v, sizeof(T), 1, f) == 1; }
+template <typename T>
+bool get(FILE *f, T &v) { return std::fread(&v, sizeof(T), 1, f) == 1; }
+template <typename T>
+bool put_vec(FILE *f, const std::vector<T> &v) {
+    const uint64_t n = v.size();
+    return put(f, n) && (n == 0 || std::fwrite(v.data(), sizeof(T), n, f) == n);
+}
+template <typename T>
+bool get_vec(FILE *f, std::vector<T> &v, uint64_t max_n) {
+    uint64_t n = 0;
+    if (!get(f, n) || n > max_n) return false;
+    v.resize(n);
+    return n == 0 || std::fread(v.data(), sizeof(T), n, f) == n;
+}
+
+}  // namespace
+
+extern "C" int acc_csr_save(const acc_csr *m, const char *path) {
+    if (!m || !path) return fail(ACC_ERR_ARG, "null argument");
+    File w(path, "wb");
+    if (!w.f) return fail(ACC_ERR_IO, std::string("cannot open ") + path + " for writing");
+    bool ok = std::fwrite(kCsrMagic, 1, 8, w.f) == 8 && put(w.f, m->rows) && put(w.f, m->cols) &&
+              put_vec(w.f, m->ptr) && put_vec(w.f, m->idx) && put_vec(w.f, m->val);
+    return ok ? ACC_OK : fail(ACC_ERR_IO, std::string("write failed: ") + path);
+}
+
+extern "C" int acc_csr_load(const char *path, acc_csr **out) {
+    if (!path || !out) return fail(ACC_ERR_ARG, "null argument");
+    File r(path, "rb");
+    if (!r.f) return fail(ACC_ERR_IO, std::string("cannot open ") + path);
+    char magic[8];
+    acc_csr *m = new acc_csr();
+    bool ok = std::fread(magic, 1, 8, r.f) == 8 && std::memcmp(magic, kCsrMagic, 8) == 0 && get(r.f, m->rows) &&
+              get(r.f, m->cols) && m->rows >= 0 && m->cols >= 0 && get_vec(r.f, m->ptr, (uint64_t)m->rows + 1) &&
+              m->ptr.size() == (size_t)m->rows + 1 && get_vec(r.f, m->idx, UINT64_MAX / 8) &&
+              get_vec(r.f, m->val, m->idx.size());
+    // the arrays must describe a CSR matrix: monotone ptr, in-range columns
+    if (ok) ok = m->ptr[0] == 0 && m->ptr[(size_t)m->rows] == (int64_t)m->idx.size() && m->val.size() == m->idx.size();
+    for (int32_t i = 0; ok && i < m->rows; ++i) ok = m->ptr[(size_t)i] <= m->ptr[(size_t)i + 1];
+    for (size_t j = 0; ok && j < m->idx.size(); ++j) ok = m->idx[j] >= 0 && m->idx[j] < m->cols;
+    if (!ok) {
+        delete m;
+        return fail(ACC_ERR_FORMAT, std::string("not a CSR file of this library: ") + path);
+    }
+    *out = m;
+    return ACC_OK;
+}
+
+extern "C" int acc_packed_save(const acc_packed *pk, const char *path) {
+    if (!pk || !path) return fail(ACC_ERR_ARG, "null argument");
+    File w(path, "wb");
+    if (!w.f) return fail(ACC_ERR_IO, std::string("cannot open ") + path + " for writing");
+    bool ok = std::fwrite(kPackMagic, 1, 8, w.f) == 8 && put(w.f, pk->rows) && put(w.f, pk->cols) &&
+              put(w.f, pk->split) && put(w.f, pk->bits) && put(w.f, pk->col_bits) && put(w.f, pk->entry_bits) &&
+              put(w.f, pk->is_signed) && put(w.f, pk->scale) && put(w.f, pk->nnz) && put(w.f, pk->nnz_kept) &&
+              put(w.f, pk->dup_rows) && put(w.f, pk->merged_max) && put(w.f, pk->merged_min) &&
+              put_vec(w.f, pk->part_row_begin) && put_vec(w.f, pk->part_slice_begin) && put_vec(w.f, pk->slice_off) &&
+              put_vec(w.f, pk->slice_ndw) && put_vec(w.f, pk->perm) && put_vec(w.f, pk->words);
+    return ok ? ACC_OK : fail(ACC_ERR_IO, std::string("write failed: ") + path);
+}
+
+extern "C" int acc_packed_load(const char *path, acc_packed **out) {
+    if (!path || !out) return fail(ACC_ERR_ARG, "null argument");
+    File r(path, "rb");
+    if (!r.f) return fail(ACC_ERR_IO, std::string("cannot open ") + path);
+    char magic[8];
+    acc_packed *pk = new acc_packed();
+    bool ok = std::fread(magic, 1, 8, r.f) == 8 && std::memcmp(magic, kPackMagic, 8) == 0 && get(r.f, pk->rows) &&
+              get(r.f, pk->cols) && get(r.f, pk->split) && get(r.f, pk->bits) && get(r.f, pk->col_bits) &&
+              get(r.f, pk->entry_bits) && get(r.f, pk->is_signed) && get(r.f, pk->scale) && get(r.f, pk->nnz) &&
+              get(r.f, pk->nnz_kept) && get(r.f, pk->dup_rows) && get(r.f, pk->merged_max) &&
+              get(r.f, pk->merged_min) && pk->split >= 1 && pk->rows >= pk->split &&
+              get_vec(r.f, pk->part_row_begin, (uint64_t)pk->split + 1) &&
+              get_vec(r.f, pk->part_slice_begin, (uint64_t)pk->split + 1) && get_vec(r.f, pk->slice_off, UINT64_MAX / 8) &&
+              get_vec(r.f, pk->slice_ndw, pk->slice_off.size()) && get_vec(r.f, pk->perm, pk->slice_off.size() * 64) &&
+              get_vec(r.f, pk->words, UINT64_MAX / 8);
+    // consistency of what the kernels index with: slices inside the words, lanes in range, widths
+    const int64_t NS = (int64_t)pk->slice_off.size();
+    if (ok) ok = pk->part_row_begin.size() == (size_t)pk->split + 1 && pk->part_slice_begin.size() == (size_t)pk->split + 1 &&
+                 pk->part_slice_begin[(size_t)pk->split] == NS && pk->slice_ndw.size() == (size_t)NS &&
+                 pk->perm.size() == (size_t)NS * 64 && pk->entry_bits == pk->col_bits + pk->bits &&
+                 pk->entry_bits >= 10 && pk->entry_bits <= 22 && pk->bits >= 2 && pk->bits <= 8;
+    for (int64_t s = 0; ok && s < NS; ++s)
+        ok = ((uint64_t)pk->slice_off[(size_t)s] + pk->slice_ndw[(size_t)s]) * 64 <= pk->words.size();
+    for (size_t i = 0; ok && i < pk->perm.size(); ++i) ok = pk->perm[i] >= -1 && pk->perm[i] < pk->rows;
+    if (!ok) {
+        delete pk;
+        return fail(ACC_ERR_FORMAT, std::string("not a packet file of this library: ") + path);
+    }
+    *out = pk;
     return ACC_OK;
 }

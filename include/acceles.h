@@ -31,7 +31,7 @@ extern "C" {
 #define ACC_OK 0
 #define ACC_ERR_ARG (-1)      /* bad argument                         */
 #define ACC_ERR_IO (-2)       /* file cannot be opened / read         */
-#define ACC_ERR_FORMAT (-3)   /* malformed Matrix Market content      */
+#define ACC_ERR_FORMAT (-3)   /* malformed file content               */
 #define ACC_ERR_RANGE (-4)    /* index or size out of range           */
 #define ACC_ERR_NOMEM (-5)    /* host or device allocation failed     */
 #define ACC_ERR_DEVICE (-6)   /* HIP runtime error                    */
@@ -190,6 +190,16 @@ int acc_packed_info(const acc_packed *pk, acc_packed_info_t *info);
 /* Decode back to CSR order: ptr[rows+1], idx[nnz_kept], qval[nnz_kept]. */
 int acc_unpack_lowbit(const acc_packed *pk, int64_t *ptr, int32_t *idx, int32_t *qval);
 void acc_packed_destroy(acc_packed *pk);
+
+/* Binary files of a CSR matrix or of packets, so a matrix is read and packed
+ * once (no counterpart: the reference's drivers re-read .mtx text and re-pack
+ * on every run, host_spmv_ucsr.cpp:278-319).  Loading checks what the kernels
+ * index with (ptr monotone, columns in range, slices inside the words, lanes
+ * in range) and fails with ACC_ERR_FORMAT otherwise. */
+int acc_csr_save(const acc_csr *csr, const char *path);
+int acc_csr_load(const char *path, acc_csr **out);
+int acc_packed_save(const acc_packed *pk, const char *path);
+int acc_packed_load(const char *path, acc_packed **out);
 
 /* ---- Synthetic inputs (create_matrices.py:57-137, seeded) ---------- */
 

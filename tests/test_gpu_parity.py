@@ -329,6 +329,20 @@ def test_dense_layout_sums_repeated_columns(acc, gpu, bits, signed):
     check_batch(acc, g, csr, bits, signed, X, ks, 256, 8)
 
 
+def test_loaded_files_give_the_same_results(acc, gpu, tmp_path):
+    csr, pk, g = make(acc, 30000, 500, (40, 100), 8, 32, seed=12)
+    csr.save(str(tmp_path / "m.csr"))
+    pk.save(str(tmp_path / "m.pk"))
+    g2 = acc.GpuMatrix(acc.CSRMatrix.load(str(tmp_path / "m.csr")), acc.PackedMatrix.load(str(tmp_path / "m.pk")))
+    X = acc.generate_queries(3, 500, seed=12)
+    for x in X:
+        a, b = empty(acc, 200, np.int32), empty(acc, 200, np.int32)
+        g.topk_lowbit(dev(acc, x), 100, 512, a)
+        g2.topk_lowbit(dev(acc, x), 100, 512, b)
+        acc.synchronize()
+        np.testing.assert_array_equal(a.numpy(), b.numpy())
+
+
 def test_fp64_exact_scores(acc, gpu):
     csr, _, g = make(acc, 20000, 500, (40, 100), seed=4)
     ptr, idx, val = csr.arrays()
