@@ -179,6 +179,28 @@ struct WaveTopM {
     // Sort, keep the best M, set the admission threshold.
     __device__ void compact() {
         const int lane = lane_id();
+        if (n <= 32) {
+            // few keys: each finds its rank by counting the keys above it (equal
+            // keys, should any occur, are ordered by lane), then moves there
+            const uint64_t k = lane < n ? buf[lane] : 0ull;
+            const uint32_t klo = (uint32_t)k, khi = (uint32_t)(k >> 32);
+            int rank = 0;
+            for (int j = 0; j < n; ++j) {
+                const uint64_t kj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)khi, j) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readlane((int)klo, j);
+                rank += (kj > k || (kj == k && j < lane)) ? 1 : 0;
+            }
+            const int nn = n < M ? n : M;
+            wave_sync();
+            if (lane < n && rank < nn) buf[rank] = k;
+            if (nn == M && M > 0) {
+                const uint64_t bm = __ballot(lane < n && rank == M - 1);
+                thr = shfl_u64(k, __builtin_ctzll(bm));
+            }
+            n = nn;
+            wave_sync();
+            return;
+        }
         if (R > 1 && n <= 64) {  // one register of keys: a 64-key sort
             uint64_t w[1] = {lane < n ? buf[lane] : 0ull};
             reg_sort_desc<1>(w);
@@ -219,6 +241,52 @@ struct WaveTopM {
         if (pred) buf[n + mbcnt64(mask)] = key;
         wave_sync();
         n += c;
+    }
+    // Offers U keys per lane at once: the lanes' counts are prefix-summed
+    // over the wavefront and the admitted keys are written to their slots in
+    // one pass.  When they do not all fit, the buffer is filled, compacted
+    // (which may raise the admission threshold) and the rest offered again.
+    template <int U>
+    __device__ void push_many(const bool (&pred)[U], const uint64_t (&key)[U]) {
+        const int lane = lane_id();
+        bool pend[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) pend[u] = pred[u];
+        for (;;) {
+            int c = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                pend[u] = pend[u] && key[u] > thr;
+                c += pend[u] ? 1 : 0;
+            }
+            int incl = c;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o);
+                if (lane >= o) incl += t;
+            }
+            const int tot = __builtin_amdgcn_readlane(incl, 63);
+            if (tot == 0) return;
+            const int room = cap - n;  // >= cap - M >= 64 after a compaction
+            int pos = incl - c;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (pend[u]) {
+                    if (pos < room) {
+                        buf[n + pos] = key[u];
+                        pend[u] = false;
+                    }
+                    ++pos;
+                }
+            }
+            wave_sync();
+            if (tot <= room) {
+                n += tot;
+                return;
+            }
+            n = cap;
+            compact();
+        }
     }
 };
 

@@ -50,8 +50,6 @@ struct BatchWs {
     size_t corr_n = 0;
     uint32_t *tk = nullptr;      // four-key slice lists
     size_t tk_n = 0;
-    uint64_t *ckeys = nullptr;   // per (query, partition) candidates
-    size_t ckeys_n = 0;
     int32_t *smax = nullptr;     // single-query passes: slice maxima
     size_t smax_n = 0;
     float *y = nullptr;          // FP32 plans: scores
@@ -433,7 +431,7 @@ int batch_layout(const acc_gpu_matrix *m, int nq) {
 }
 
 // Batched low-bit queries on the dense tiles: dense_prep, dense_scores,
-// dense_select, then (out != null) batch_rescore and batch_topk.
+// dense_select (which also rescores when out != null), then batch_topk.
 int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, int kmax, acc_element *out,
                  int out_stride, int32_t *crow, int32_t *cscore, hipStream_t st) {
     int rc = build_dense(m);
@@ -447,7 +445,6 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     if (rc == ACC_OK) rc = ws_grow(ws, &ws.qplain, &ws.qplain_n, (size_t)nq * m->cols_pad);
     if (rc == ACC_OK) rc = ws_grow(ws, &ws.corr, &ws.corr_n, (size_t)nqb * QB);
     if (rc == ACC_OK) rc = ws_grow(ws, &ws.tk, &ws.tk_n, (size_t)nq * m->nslices * 4);
-    if (rc == ACC_OK) rc = ws_grow(ws, &ws.ckeys, &ws.ckeys_n, cand);
     if (rc == ACC_OK) rc = ws_grow(ws, &ws.resc, &ws.resc_n, cand);
     if (rc != ACC_OK) return rc;
     DenseArgs a = dense_args(m);
@@ -460,7 +457,6 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     a.corr = ws.corr;
     a.tk = ws.tk;
     a.M = M;
-    a.ckeys = ws.ckeys;
     a.resc = ws.resc;
     a.K = ws.K;
     a.out = out;
@@ -481,19 +477,17 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     const int nwx = (int)std::max<int64_t>(8, std::min<int64_t>(groups, std::max(1, m->cus / nqb)) / 8 * 8);
     LAUNCH(m, 1, fs, dim3(nwx * nqb), dim3(512), lds_s, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
-    const size_t lds_sel = (size_t)ACC_WAVES * 256 * 8 + (size_t)ACC_WAVES * m->cols_pad;
+    // per wavefront: two key buffers, the int8 query (rescans), the FP32 query (rescoring)
+    const size_t lds_sel = (size_t)ACC_WAVES * 256 * 8 + (size_t)ACC_WAVES * m->cols_pad * 5;
     if ((rc = lds_limit((const void *)get_dense_select(), lds_sel)) != ACC_OK) return rc;
     LAUNCH(m, 2, get_dense_select(), dim3(m->split, (nq + ACC_WAVES - 1) / ACC_WAVES), dim3(ACC_WG), lds_sel, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
     if (!out) return ACC_OK;
-    const int64_t threads = (int64_t)cand * m->group;
-    LAUNCH(m, 3, get_batch_rescore(), dim3((unsigned)((threads + ACC_WG - 1) / ACC_WG)), dim3(ACC_WG), 0, st, a);
-    if ((rc = check_launch()) != ACC_OK) return rc;
     const int total = m->split * M;  // <= kMaxCK = 4096 = 4 lists of 64*16
     const int R = pow2_at_least((total + 64 * ACC_WAVES - 1) / (64 * ACC_WAVES));
     dense_fn ft = kmax <= 256 ? get_batch_topk_radix(R) : get_batch_topk(R);
     if (!ft) return fail(ACC_ERR_ARG, "too many candidates");
-    LAUNCH(m, 4, ft, dim3(nq), dim3(ACC_WG), kmax <= 256 ? 0 : (size_t)ACC_WAVES * 64 * R * 8, st, a);
+    LAUNCH(m, 3, ft, dim3(nq), dim3(ACC_WG), kmax <= 256 ? 0 : (size_t)ACC_WAVES * 64 * R * 8, st, a);
     return check_launch();
 }
 

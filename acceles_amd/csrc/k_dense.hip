@@ -341,35 +341,51 @@ __device__ __forceinline__ void reg_sort_desc_u32(uint32_t (&v)[R]) {
     }
 }
 
-// Offers the listed keys of 64*U slices (k4[u] = the list of slice
-// s0 + b + u*64 + lane) scoring >= ts to the collector; a slice whose list
-// may have dropped such a row is rescanned exactly (from the dense tiles).
-// nv[u]: valid rows of those slices (0 past the partition); keys of padding
-// rows (row_in_slice >= nv) are discarded, and since they may have displaced
-// real rows from a list, every list whose fourth key scores >= ts is rescanned.
+// Global rows of the listed keys of 64*U slices (k4[u] = the list of slice
+// s0 + b + u*64 + lane); -1 for keys of padding rows (row_in_slice >= nv[u],
+// nv[u] = 0 past the partition).  Issued before the threshold is known, so
+// the lookups are in flight while it is computed.
 template <int U>
-__device__ __forceinline__ void offer_lists(const DenseArgs &a, WaveTopM<2> &cb, const uint4 (&k4)[U], const int (&nv)[U],
-                                   int64_t s0, int b, int32_t ts, int q, int8_t *qx, bool &have_qx) {
+__device__ __forceinline__ void list_rows(const DenseArgs &a, const uint4 (&k4)[U], const int (&nv)[U], int64_t s0,
+                                          int b, int32_t (&row)[U][4]) {
     const int lane = lane_id();
-    bool rescan[U];
-    int32_t row[U][4];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int64_t s = s0 + b + u * 64 + lane;
+        const uint32_t ke[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            // loaded for every key (slices past the matrix clamped into it) so
+            // that the loads issue together; the row is kept for real rows
+            const int32_t r = a.perm[min(s, a.nslices - 1) * 64 + tk_rho(ke[e])];
+            row[u][e] = tk_rho(ke[e]) < nv[u] ? r : -1;
+        }
+    }
+}
+
+// Offers the listed keys of those slices scoring >= ts to the collector; a
+// slice whose list may have dropped such a row is rescanned exactly (from the
+// dense tiles).  Keys of padding rows are discarded, and since they may have
+// displaced real rows from a list, every list whose fourth key scores >= ts
+// is rescanned.
+template <int U>
+__device__ __forceinline__ void offer_lists(const DenseArgs &a, WaveTopM<2> &cb, const uint4 (&k4)[U], const int (&nv)[U],
+                                   const int32_t (&row)[U][4], int64_t s0, int b, int32_t ts, int q, int8_t *qx,
+                                   bool &have_qx) {
+    const int lane = lane_id();
+    bool rescan[U], pass[U * 4];
+    uint64_t key[U * 4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
         rescan[u] = nv[u] > 0 && tk_score(k4[u].w) >= ts;
         const uint32_t ke[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const bool pass = !rescan[u] && nv[u] > 0 && tk_rho(ke[e]) < nv[u] && tk_score(ke[e]) >= ts;
-            row[u][e] = pass ? a.perm[s * 64 + tk_rho(ke[e])] : -1;
+            pass[u * 4 + e] = !rescan[u] && row[u][e] >= 0 && tk_score(ke[e]) >= ts;
+            key[u * 4 + e] = dkey_i32(tk_score(ke[e]), row[u][e]);
         }
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const uint32_t ke[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
-#pragma unroll
-        for (int e = 0; e < 4; ++e) cb.push(row[u][e] >= 0, dkey_i32(tk_score(ke[e]), row[u][e]));
-    }
+    cb.push_many<U * 4>(pass, key);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         uint64_t mask = __ballot(rescan[u]);
@@ -390,20 +406,62 @@ __device__ __forceinline__ void offer_lists(const DenseArgs &a, WaveTopM<2> &cb,
     }
 }
 
+// Stage-1 outputs of one (query, partition) and its FP32 rescoring
+// (quantify_util.py:354-367): keys[0, nn) are the partition's Top-M in rank
+// order (LDS); slot i of the candidate list is keys[i] or empty past nn.
+// Each group of G lanes rescores C candidates at once in the declared order
+// (strided fmaf, then the butterfly), so the scores equal those of every
+// other rescoring path; xs is the FP32 query, staged in LDS.
+__device__ __forceinline__ void emit_candidates(const DenseArgs &a, const uint64_t *keys, int nn, size_t base,
+                                                const float *xs) {
+    const int lane = lane_id(), M = a.M;
+    if (a.cand_rows_out)
+        for (int i = lane; i < M; i += 64) {
+            const uint64_t key = i < nn ? keys[i] : 0ull;
+            a.cand_rows_out[base + i] = key ? dkey_row(key) : -1;
+            a.cand_scores_out[base + i] = key ? dkey_score_i32(key) : 0;
+        }
+    if (!a.out) return;
+    constexpr int C = 4;
+    const int G = a.G, groups = 64 / G, grp = lane / G;
+    wave_sync();
+    for (int i0 = 0; i0 < M; i0 += groups * C) {
+        int32_t r[C];
+        bool valid[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int i = i0 + c * groups + grp;
+            const uint64_t key = keys[min(i, 63)];
+            valid[c] = i < nn;
+            r[c] = valid[c] ? dkey_row(key) : 0;
+        }
+        float sc[C];
+        group_dot_rt_multi<C, 4>(a.ptr, a.idx, a.val, xs, r, valid, G, sc);
+        if ((lane & (G - 1)) == 0)
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int i = i0 + c * groups + grp;
+                if (i < M) a.resc[base + i] = valid[c] ? dkey_f32(sc[c], r[c]) : 0ull;
+            }
+    }
+}
+
 // One wavefront per (partition p = blockIdx.x, query q = blockIdx.y*4 + wave):
 //  1. ts = the M-th best slice maximum of the partition (INT32_MIN when the
-//     partition has at most M slices).  M distinct rows score >= ts, so the
-//     partition's M-th best score is >= ts and no row below ts can be in its
-//     Top-M.
+//     partition has fewer than M full slices).  M distinct rows score >= ts,
+//     so the partition's M-th best score is >= ts and no row below ts can be
+//     in its Top-M.
 //  2. Every listed key of a real row scoring >= ts is a candidate.  A list
 //     whose fourth key still scores >= ts may have dropped a qualifying row:
-//     that slice is rescanned exactly (all 64 rows, one dot4 pass) instead.  Slice maxima
-//     of partial slices (padding rows score 0) are left out of step 1; the
-//     maxima of the full slices still give M distinct real rows >= ts.
+//     that slice is rescanned exactly (all 64 rows, one dot4 pass) instead.
+//     Slice maxima of partial slices (padding rows score 0) are left out of
+//     step 1; the maxima of the full slices still give M distinct real rows
+//     >= ts.
 //  3. Top-M of the candidates by (score, row) (quantify_util.py:173-196 with
-//     the lower-row tie rule), written as keys for rescoring.
-// Partitions of at most 256 slices keep their lists in registers and find ts
-// with one register sort; larger ones stream the lists twice.
+//     the lower-row tie rule), then their FP32 rescoring (out != null).
+// Partitions of at most 256 slices (M <= 32, at most 64 candidates) keep
+// their lists in registers: one prefix sum appends the candidates, a count
+// ranks them.  Anything else streams the lists twice through a collector.
 __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x >> 6, lane = lane_id();
@@ -411,15 +469,16 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
     if (q >= a.nq) return;  // whole wavefront; no workgroup barrier below
     uint64_t *bufA = (uint64_t *)smem + wave * 256, *bufB = bufA + 128;
     int8_t *qx = (int8_t *)(smem + ACC_WAVES * 256 * 8) + (size_t)wave * a.cols_pad;
+    float *xs = (float *)(smem + ACC_WAVES * 256 * 8 + (size_t)ACC_WAVES * a.cols_pad) + (size_t)wave * a.cols_pad;
+    const float *xq = a.X + (size_t)q * a.cols;
     const int64_t s0 = a.part_slice[p];
     const int nsl = (int)(a.part_slice[p + 1] - s0);
     const uint4 *L = (const uint4 *)a.tk + (size_t)q * a.nslices + s0;
     const int M = a.M;
-    WaveTopM<2> cb;
-    cb.init(bufB, M);
+    const size_t base = ((size_t)q * a.split + p) * M;
     bool have_qx = false;
 
-    if (nsl <= 256) {
+    if (nsl <= 256 && M <= 32) {
         uint4 k4[4];
         int nv[4];
 #pragma unroll
@@ -428,90 +487,138 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
             k4[u] = i < nsl ? L[i] : make_uint4(0u, 0u, 0u, 0u);
             nv[u] = i < nsl ? (int)a.nvalid[s0 + i] : 0;
         }
+        // the query for the rescoring, staged while the lists arrive
+        if (a.out)
+            for (int cc = lane; cc < a.cols; cc += 64) xs[cc] = xq[cc];
+        int32_t row[4][4];
+        list_rows<4>(a, k4, nv, s0, 0, row);
         int32_t ts = INT32_MIN;
-        if (nsl > M) {
+        {
             uint32_t v[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) v[u] = nv[u] == 64 ? k4[u].x : 0u;
-            uint32_t t = 0u;
-            if (M <= 32) {
-                t = wave_kth_u32<4>(v, M);
-            } else {
-                reg_sort_desc_u32<4>(v);
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (r == (M - 1) >> 6) t = v[r];
-                t = (uint32_t)__shfl((int)t, (M - 1) & 63);
-            }
+            const uint32_t t = wave_kth_u32<4>(v, M);
             if (t != 0u) ts = tk_score(t);
         }
-        offer_lists<4>(a, cb, k4, nv, s0, 0, ts, q, qx, have_qx);
-    } else {
-        constexpr int U = 4;
-        int32_t ts = INT32_MIN;
-        if (nsl <= 1024 && M <= 32) {  // the M-th largest maximum by extraction, maxima in registers
-            uint32_t v[16];
+        bool rescan[4], pass[16];
+        int c = 0;
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int i = u * 64 + lane;
-                v[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
+        for (int u = 0; u < 4; ++u) {
+            rescan[u] = nv[u] > 0 && tk_score(k4[u].w) >= ts;
+            const uint32_t ke[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                pass[u * 4 + e] = !rescan[u] && row[u][e] >= 0 && tk_score(ke[e]) >= ts;
+                c += pass[u * 4 + e] ? 1 : 0;
             }
-            const uint32_t t = wave_kth_u32<16>(v, M);
-            if (t != 0u) ts = tk_score(t);
-        } else {
-            WaveTopM<2> ca;
-            ca.init(bufA, M);
-            for (int b = 0; b < nsl; b += 64 * U) {
-                uint32_t k[U];
+        }
+        int incl = c;
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int i = b + u * 64 + lane;
-                    k[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        int n = __builtin_amdgcn_readlane(incl, 63);
+        if (n <= 64) {
+            int pos = incl - c;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t ke[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (pass[u * 4 + e]) bufB[pos++] = dkey_i32(tk_score(ke[e]), row[u][e]);
+            }
+            // slices to rescan, listed in LDS so that one loop serves them all
+            uint32_t *rl = (uint32_t *)bufA;
+            int nr = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint64_t m = __ballot(rescan[u]);
+                if (rescan[u]) rl[nr + mbcnt64(m)] = (uint32_t)(u * 64 + lane);
+                nr += __popcll(m);
+            }
+            wave_sync();
+            for (int k = 0; k < nr && n <= 64; ++k) {
+                const int64_t s = s0 + rl[k];
+                if (!have_qx) {
+                    const int8_t *src = a.qplain + (size_t)q * a.cols_pad;
+                    for (int cc = lane; cc < a.cols_pad; cc += 64) qx[cc] = src[cc];
+                    wave_sync();
+                    have_qx = true;
                 }
-#pragma unroll
-                for (int u = 0; u < U; ++u) ca.push(k[u] != 0u, ((uint64_t)k[u] << 32) | (uint32_t)(b + u * 64 + lane));
+                const int32_t sc = slice_dot_dense(a, s, qx, a.corr[q]);
+                const int32_t r = a.perm[s * 64 + lane];
+                const bool pr = r >= 0 && sc >= ts;
+                const uint64_t bm = __ballot(pr);
+                if (n + __popcll(bm) <= 64 && pr) bufB[n + mbcnt64(bm)] = dkey_i32(sc, r);
+                n += __popcll(bm);
             }
-            ca.compact();
-            if (ca.n >= M) ts = tk_score((uint32_t)(ca.buf[M - 1] >> 32));
+            if (n <= 64) {
+                wave_sync();
+                const uint64_t k = lane < n ? bufB[lane] : 0ull;
+                const uint32_t klo = (uint32_t)k, khi = (uint32_t)(k >> 32);
+                int rank = 0;
+                for (int j = 0; j < n; ++j) {
+                    const uint64_t kj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)khi, j) << 32) |
+                                        (uint32_t)__builtin_amdgcn_readlane((int)klo, j);
+                    rank += (kj > k || (kj == k && j < lane)) ? 1 : 0;
+                }
+                const int nn = n < M ? n : M;
+                if (lane < n && rank < nn) bufA[rank] = k;
+                wave_sync();
+                emit_candidates(a, bufA, nn, base, xs);
+                return;
+            }
         }
+    }
+
+    // General path: the lists are streamed twice (threshold, then offers).
+    if (a.out)
+        for (int cc = lane; cc < a.cols; cc += 64) xs[cc] = xq[cc];
+    WaveTopM<2> cb;
+    cb.init(bufB, M);
+    constexpr int U = 4;
+    int32_t ts = INT32_MIN;
+    if (nsl <= 1024 && M <= 32) {  // the M-th largest maximum by extraction, maxima in registers
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int i = u * 64 + lane;
+            v[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
+        }
+        const uint32_t t = wave_kth_u32<16>(v, M);
+        if (t != 0u) ts = tk_score(t);
+    } else {
+        WaveTopM<2> ca;
+        ca.init(bufA, M);
         for (int b = 0; b < nsl; b += 64 * U) {
-            uint4 k4[U];
-            int nv[U];
+            uint32_t k[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = b + u * 64 + lane;
-                k4[u] = i < nsl ? L[i] : make_uint4(0u, 0u, 0u, 0u);
-                nv[u] = i < nsl ? (int)a.nvalid[s0 + i] : 0;
+                k[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
             }
-            offer_lists<U>(a, cb, k4, nv, s0, b, ts, q, qx, have_qx);
+#pragma unroll
+            for (int u = 0; u < U; ++u) ca.push(k[u] != 0u, ((uint64_t)k[u] << 32) | (uint32_t)(b + u * 64 + lane));
         }
+        ca.compact();
+        if (ca.n >= M) ts = tk_score((uint32_t)(ca.buf[M - 1] >> 32));
+    }
+    for (int b = 0; b < nsl; b += 64 * U) {
+        uint4 k4[U];
+        int nv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = b + u * 64 + lane;
+            k4[u] = i < nsl ? L[i] : make_uint4(0u, 0u, 0u, 0u);
+            nv[u] = i < nsl ? (int)a.nvalid[s0 + i] : 0;
+        }
+        int32_t row[U][4];
+        list_rows<U>(a, k4, nv, s0, b, row);
+        offer_lists<U>(a, cb, k4, nv, row, s0, b, ts, q, qx, have_qx);
     }
     cb.compact();
-    const size_t base = ((size_t)q * a.split + p) * M;
-    for (int i = lane; i < M; i += 64) {
-        const uint64_t key = i < cb.n ? cb.buf[i] : 0ull;
-        a.ckeys[base + i] = key;
-        if (a.cand_rows_out) {
-            a.cand_rows_out[base + i] = key ? dkey_row(key) : -1;
-            a.cand_scores_out[base + i] = key ? dkey_score_i32(key) : 0;
-        }
-    }
-}
-
-// FP32 rescoring of every candidate (quantify_util.py:354-367): G lanes per
-// candidate in the declared strided-fmaf + butterfly order.
-__global__ __launch_bounds__(ACC_WG) void batch_rescore(DenseArgs a) {
-    const int G = a.G;
-    const int64_t per_q = (int64_t)a.split * a.M;
-    const int64_t n = (int64_t)a.nq * per_q;
-    const int64_t cid = ((int64_t)blockIdx.x * ACC_WG + threadIdx.x) / G;
-    const bool in = cid < n;
-    const uint64_t key = in ? a.ckeys[cid] : 0ull;
-    const bool valid = key != 0ull;
-    const int32_t r = valid ? dkey_row(key) : 0;
-    const int q = in ? (int)(cid / per_q) : 0;
-    const float sc = group_dot_rt(a.ptr, a.idx, a.val, a.X + (size_t)q * a.cols, r, valid, G);
-    if (in && (threadIdx.x & (G - 1)) == 0) a.resc[cid] = valid ? dkey_f32(sc, r) : 0ull;
+    emit_candidates(a, cb.buf, cb.n, base, xs);
 }
 
 // One workgroup per query: each wavefront sorts a quarter of the query's
@@ -551,7 +658,6 @@ dense_fn get_dense_scores(int NT, bool stream) {
     }
 }
 dense_fn get_dense_select() { return dense_select; }
-dense_fn get_batch_rescore() { return batch_rescore; }
 dense_fn get_batch_topk_radix(int V) {
     switch (V) {
         case 1: return batch_topk_radix<1>;

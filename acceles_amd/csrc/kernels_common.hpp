@@ -173,9 +173,11 @@ __device__ __forceinline__ float group_dot_rt(const uint32_t *ptr, const int32_t
             int32_t c[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t jj = j + u * G;
-                v[u] = jj < e ? val[jj] : 0.f;
-                c[u] = jj < e ? idx[jj] : 0;
+                // clamped into the row (j < e): unconditional loads issue
+                // together, where conditional ones would each be waited for
+                const uint32_t jj = min(j + u * G, e - 1);
+                v[u] = val[jj];
+                c[u] = idx[jj];
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
@@ -186,6 +188,59 @@ __device__ __forceinline__ float group_dot_rt(const uint32_t *ptr, const int32_t
     }
     for (int s = G >> 1; s >= 1; s >>= 1) p = p + __shfl_xor(p, s);
     return p;
+}
+
+// The same dot products for C rows at once (rows r[c] against x): their
+// loads are issued together, so C rows cost the memory round trips of one.
+// Each row is summed in the order of group_dot_rt (entries g, g+G, ... with
+// fmaf, then the butterfly), so the results are the same floats.
+template <int C, int U = 4>
+__device__ __forceinline__ void group_dot_rt_multi(const uint32_t *ptr, const int32_t *idx, const float *val,
+                                                   const float *x, const int32_t (&r)[C],
+                                                   const bool (&valid)[C], int G, float (&out)[C]) {
+    const int g = lane_id() & (G - 1);
+    uint32_t b[C], e[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        // r[c] is a row of the matrix also for invalid candidates, so the
+        // loads are unconditional and issue together
+        const uint32_t p0 = ptr[r[c]], p1 = ptr[r[c] + 1];
+        b[c] = valid[c] ? p0 + g : 0u;
+        e[c] = valid[c] ? p1 : 0u;
+        out[c] = 0.f;
+    }
+    for (uint32_t o = 0;; o += U * G) {
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < C; ++c) any = any || b[c] + o < e[c];
+        if (!any) break;
+        float v[C][U], xv[C][U];
+        int32_t cl[C][U];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            // clamped into the matrix (some row has entries while the loop
+            // runs): unconditional loads issue together
+            const uint32_t last = e[c] > 0u ? e[c] - 1u : 0u;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t jj = min(b[c] + o + u * G, last);
+                v[c][u] = val[jj];
+                cl[c][u] = idx[jj];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[c][u] = x[cl[c][u]];
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (b[c] + o + u * G < e[c]) out[c] = __builtin_fmaf(v[c][u], xv[c][u], out[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        for (int s = G >> 1; s >= 1; s >>= 1) out[c] = out[c] + __shfl_xor(out[c], s);
 }
 
 
@@ -516,8 +571,8 @@ struct QTable {
 //                  best (score, row-in-slice) keys;
 //   dense_select   per (partition, query): threshold = the M-th best slice
 //                  maximum, candidates from the four-key lists, exact rescan of
-//                  any slice whose list may have dropped a candidate, Top-M;
-//   batch_rescore  FP32 rescoring of every (query, candidate);
+//                  any slice whose list may have dropped a candidate, Top-M,
+//                  and the FP32 rescoring of those M rows;
 //   batch_topk     per query: its Top-K[q] of the rescored keys.
 //
 // Tile layout: tile (rt, kb) of slice s (rows s*64 + rt*32 .. +31, columns
@@ -545,7 +600,6 @@ struct DenseArgs {
     int32_t *corr;               // nqb*32*NT per-query offsets
     uint32_t *tk;                // nq x nslices x 4 keys
     int split, M;
-    uint64_t *ckeys;             // nq x split x M (score, row) keys, 0 = none
     const uint32_t *ptr;         // FP32 CSR for rescoring
     const int32_t *idx;
     const float *val;
@@ -592,6 +646,5 @@ dense_fn get_dense_build();                                    // k_dense.hip
 dense_fn get_dense_prep();
 dense_fn get_dense_scores(int NT, bool stream);
 dense_fn get_dense_select();
-dense_fn get_batch_rescore();
 dense_fn get_batch_topk_radix(int V);
 dense_fn get_batch_topk(int R);

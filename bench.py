@@ -468,10 +468,10 @@ def main():
     lb_ms = np.array([g.time_lowbit_stage(xs[0], K, CK, st_, 40, stream) for st_ in range(3)])
     kept = pk.info.nnz_kept
     nslots = pk.info.nslices * 64
-    dense = len(bt_ms) == 5
+    dense = len(bt_ms) == 4
     if dense:
-        # dense int8 tiles on the matrix cores: prep, scores, select, rescore, final
-        names = ["prep", "scores", "select", "rescore", "final"]
+        # dense int8 tiles on the matrix cores: prep, scores, select + FP32 rescoring, final
+        names = ["prep", "scores", "select_rescore", "final"]
         t_sc = bt_ms[1]
         cols_pad = (a.cols + 31) // 32 * 32
         bs_bytes = g.info.dense_bytes + B * cols_pad + B * pk.info.nslices * 16

@@ -267,7 +267,7 @@ int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info);
  * kernel's own start and end); acc_gpu_stage_times waits for the last query
  * and returns up to n kernel durations in ms (low-bit: slice maxima, select
  * + rescoring, final Top-K; batched packets: scores, select, final; batched
- * dense tiles: prep, scores, select, rescoring, final; FP32: SpMV, select,
+ * dense tiles: prep, scores, select + rescoring, final; FP32: SpMV, select,
  * final select) and their count. */
 int acc_gpu_set_profiling(acc_gpu_matrix *m, int enable);
 int acc_gpu_stage_times(acc_gpu_matrix *m, float *ms, int n, int *count);
@@ -316,9 +316,10 @@ int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq,
  * dense int8 tiles on the matrix cores (v_mfma_i32_32x32x32_i8; 64-row slices
  * x columns rounded up to 32, one byte per entry).  Both give the same
  * integer scores and the same results.  ACC_BATCH_DENSE fails with a message
- * when the layout cannot hold the matrix exactly (a row naming a column twice,
- * |score| >= 2^25, or more than 4960 columns); ACC_BATCH_AUTO uses the dense
- * tiles when they can hold the matrix and the batch has at least
+ * when the layout cannot hold the matrix exactly (a row whose values in one
+ * column sum past one byte, |score| >= 2^25, or more than 4960 columns);
+ * ACC_BATCH_AUTO uses the dense tiles when they can hold the matrix and the
+ * batch has at least
  * min_dense_batch queries (0 keeps the current value; default 16), building
  * them on first use.  Setting DENSE or AUTO on an eligible matrix builds the
  * tiles now. */
