@@ -50,8 +50,9 @@ def parse():
     ap.add_argument("--batch", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--inflight", type=int, default=2, choices=[1, 2, 3, 4],
-                    help="batches (embedding) or queries (large-single) in flight on as many streams")
+    ap.add_argument("--inflight", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="batches (embedding, batched) or queries (large-single) in flight on as many streams; "
+                         "0: 3 batches, 2 queries")
     ap.add_argument("--batch-mode", choices=["auto", "packets", "dense"], default="auto",
                     help="scoring layout of the batched path")
     ap.add_argument("--workload", choices=["embedding", "large-single", "batched", "serve", "copy", "resparse"],
@@ -156,7 +157,7 @@ def bench_large_single(a, acc, stream):
     dt_seq = timed(stream, step, a.steps, a.warmup)
     # queries in flight: HIP-graph plans with their own workspaces on as many streams, so one query's
     # selection kernels (a few dozen workgroups) overlap the next query's streaming pass
-    NF = a.inflight
+    NF = a.inflight or 2
     streams = [stream] + [acc.Stream(0) for _ in range(1, NF)]
     plans = [acc.Plan(g, acc.PLAN_LOWBIT, xs[j], [K], CK, os_[j], K) for j in range(NF)]
 
@@ -228,7 +229,7 @@ def bench_batched(a, acc, stream):
     dt1 = timed(stream, lambda: plan.launch(stream), a.steps, a.warmup)
     res = out.numpy()
     # batches in flight: one plan (own workspace) per stream
-    NF = a.inflight
+    NF = a.inflight or 3
     streams = [stream] + [acc.Stream(0) for _ in range(1, NF)]
     xs2 = [Xd] + [acc.DeviceBuffer.from_numpy(acc.generate_queries(B, cols, seed=0xACCE1E5 + 60 + j))
                   for j in range(1, NF)]
@@ -413,9 +414,9 @@ def main():
     qps1 = a.steps * B / dt1
     res = out.numpy()
 
-    # ---- timed region: two batches in flight (HIP-graph plans with their own workspaces on two streams):
-    # one batch's selection, rescoring and Top-K kernels overlap the next batch's tile pass ----
-    NF = a.inflight
+    # ---- timed region: batches in flight (HIP-graph plans with their own workspaces on as many
+    # streams): one batch's selection, rescoring and Top-K kernels overlap the next batch's tile pass ----
+    NF = a.inflight or 3
     Xs = [Xd] + [acc.DeviceBuffer.from_numpy(acc.generate_queries(B, a.cols, seed=0xACCE1E5 + 7 + j))
                  for j in range(1, NF)]
     outs2 = [acc.DeviceBuffer((B, K, 2), np.int32) for _ in range(NF)]
@@ -542,7 +543,7 @@ def main():
         "config": {
             "workload": f"low-bit two-stage Top-{K}, batched: {a.rows}x{a.cols}, row nnz U[{a.deg[0]},{a.deg[1]}], "
                         f"|N(0,1)| L2-normalised, {a.bits}-bit packets, SPLIT={a.split}, CK={CK}, "
-                        f"{B} N(0,1) queries per step, {a.inflight} steps in flight (graph plans on as many streams)",
+                        f"{B} N(0,1) queries per step, {NF} steps in flight (graph plans on as many streams)",
             "rows": a.rows, "cols": a.cols, "nnz": csr.nnz, "bits": a.bits, "split": a.split, "ck": CK,
             "k": K, "queries_per_step": B, "parallelism": "single GPU",
         },
