@@ -406,6 +406,21 @@ __device__ __forceinline__ void offer_lists(const DenseArgs &a, WaveTopM<2> &cb,
     }
 }
 
+// Copies the FP32 query (cols values) into this wavefront's LDS: eight
+// clamped loads per lane are issued together per 512 columns (a loop of
+// single loads would wait for each one before its LDS write).
+__device__ __forceinline__ void stage_query(float *xs, const float *xq, int cols) {
+    const int lane = lane_id();
+    for (int c0 = 0; c0 < cols; c0 += 512) {
+        float t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = xq[min(c0 + u * 64 + lane, cols - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c0 + u * 64 + lane < cols) xs[c0 + u * 64 + lane] = t[u];
+    }
+}
+
 // Stage-1 outputs of one (query, partition) and its FP32 rescoring
 // (quantify_util.py:354-367): keys[0, nn) are the partition's Top-M in rank
 // order (LDS); slot i of the candidate list is keys[i] or empty past nn.
@@ -478,18 +493,36 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
     const size_t base = ((size_t)q * a.split + p) * M;
     bool have_qx = false;
 
-    if (nsl <= 256 && M <= 32) {
+    if (nsl > 0 && nsl <= 256 && M <= 32) {
+        // unconditional loads (clamped into the partition, masked after) so
+        // that the lists and the query for the rescoring arrive in one round
+        // trip; a conditional load would be waited for on its own
+        const bool narrow = a.cols <= 512;
+        float xt[8];
+        if (a.out && narrow)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) xt[u] = xq[min(u * 64 + lane, a.cols - 1)];
         uint4 k4[4];
         int nv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int i = u * 64 + lane;
-            k4[u] = i < nsl ? L[i] : make_uint4(0u, 0u, 0u, 0u);
-            nv[u] = i < nsl ? (int)a.nvalid[s0 + i] : 0;
+            const int i = min(u * 64 + lane, nsl - 1);
+            k4[u] = L[i];
+            nv[u] = (int)a.nvalid[s0 + i];
         }
-        // the query for the rescoring, staged while the lists arrive
-        if (a.out)
-            for (int cc = lane; cc < a.cols; cc += 64) xs[cc] = xq[cc];
+        if (a.out && narrow) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (u * 64 + lane < a.cols) xs[u * 64 + lane] = xt[u];
+        } else if (a.out) {
+            stage_query(xs, xq, a.cols);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (u * 64 + lane >= nsl) {
+                k4[u] = make_uint4(0u, 0u, 0u, 0u);
+                nv[u] = 0;
+            }
         int32_t row[4][4];
         list_rows<4>(a, k4, nv, s0, 0, row);
         int32_t ts = INT32_MIN;
@@ -573,8 +606,7 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
     }
 
     // General path: the lists are streamed twice (threshold, then offers).
-    if (a.out)
-        for (int cc = lane; cc < a.cols; cc += 64) xs[cc] = xq[cc];
+    if (a.out) stage_query(xs, xq, a.cols);
     WaveTopM<2> cb;
     cb.init(bufB, M);
     constexpr int U = 4;
