@@ -155,11 +155,21 @@ __global__ __launch_bounds__(SEL_WG) void lowbit_select(LowbitArgs a) {
     int32_t sm[V];
     bool full[V];
     if (regs) {
+        // one uniform test around loads with clamped indices: they issue
+        // together (a per-element condition would wait for each load)
+        if (nsl > 0) {
 #pragma unroll
-        for (int u = 0; u < V; ++u) {
-            const int i = min(tid + u * SEL_WG, nsl - 1);
-            sm[u] = nsl > 0 ? smax[i] : 0;
-            full[u] = nsl > 0 && nvalid[i] == 64;
+            for (int u = 0; u < V; ++u) {
+                const int i = min(tid + u * SEL_WG, nsl - 1);
+                sm[u] = smax[i];
+                full[u] = nvalid[i] == 64;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+                sm[u] = 0;
+                full[u] = false;
+            }
         }
     }
     if (tid == 0) {
