@@ -133,20 +133,34 @@ __global__ __launch_bounds__(ACC_WG) void dense_prep(DenseArgs a) {
     const int wave = threadIdx.x >> 6, lane = lane_id();
     const bool valid = q < a.nq;
     const float *x = a.X + (size_t)(valid ? q : 0) * a.cols;
+    // the first XR*ACC_WG columns are read once, with loads issued together
+    // (clamped indices, masked after), and kept in registers for both passes
+    constexpr int XR = 4;
+    float xr[XR];
+#pragma unroll
+    for (int u = 0; u < XR; ++u) xr[u] = x[min((int)threadIdx.x + u * ACC_WG, a.cols - 1)];
     float m = 0.f;
+#pragma unroll
+    for (int u = 0; u < XR; ++u)
+        if (valid && (int)threadIdx.x + u * ACC_WG < a.cols) m = fmaxf(m, fabsf(xr[u]));
     if (valid)
-        for (int c = threadIdx.x; c < a.cols; c += ACC_WG) m = fmaxf(m, fabsf(x[c]));
+        for (int c = threadIdx.x + XR * ACC_WG; c < a.cols; c += ACC_WG) m = fmaxf(m, fabsf(x[c]));
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
     if (lane == 0) red[wave] = m;
     __syncthreads();
     float s = red[0];
     for (int w = 1; w < ACC_WAVES; ++w) s = fmaxf(s, red[w]);
     int sum = 0;
-    for (int c = threadIdx.x; c < a.cols_pad; c += ACC_WG) {
+    for (int c = threadIdx.x, it = 0; c < a.cols_pad; c += ACC_WG, ++it) {
         int qv = 0;
         if (valid && c < a.cols && s > 0.f) {
-            const float u = __fmul_rn(127.0f, __fdiv_rn(x[c], s));
-            qv = (int)__builtin_rintf(u);
+            float xc = 0.f;
+#pragma unroll
+            for (int u = 0; u < XR; ++u)
+                if (it == u) xc = xr[u];
+            if (it >= XR) xc = x[c];
+            const float qf = __fmul_rn(127.0f, __fdiv_rn(xc, s));
+            qv = (int)__builtin_rintf(qf);
             qv = qv > 127 ? 127 : (qv < -128 ? -128 : qv);
         }
         const int kb = c >> 5, h = (c >> 4) & 1, bb = c & 15;
