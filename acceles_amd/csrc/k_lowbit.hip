@@ -288,9 +288,10 @@ __global__ __launch_bounds__(SEL_WG) void lowbit_select(LowbitArgs a) {
             constexpr int PR = CS / SEL_WAVES;  // slices each wavefront offers
             int32_t prow[PR];
 #pragma unroll
-            for (int r = 0; r < PR; ++r) {
+            for (int r = 0; r < PR; ++r) {  // unconditional (clamped) lookups, masked after
                 const int c = wave + r * SEL_WAVES;
-                prow[r] = c < cn ? a.perm[(s0 + s_list[cb + c]) * 64 + lane] : -1;
+                const int32_t v = a.perm[(s0 + s_list[cb + min(c, cn - 1)]) * 64 + lane];
+                prow[r] = c < cn ? v : -1;
             }
             __syncthreads();
             const int nu = s_ustart[cn];

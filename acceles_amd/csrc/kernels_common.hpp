@@ -35,20 +35,35 @@
 
 // Quantize the FP32 query into LDS: q = clip(rint(127 * (x / max|x|)), -128, 127)
 // (quantify_vector with B=8, symmetric; float32 division and product, half-even).
+// The first 4*blockDim.x values are read once, with loads issued together
+// (clamped indices, masked after), and kept in registers for both passes.
 __device__ inline void quantize_query_lds(const float *x, int cols, int8_t *qx, float *red) {
+    constexpr int XR = 4;
+    const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
+    float xr[XR];
+#pragma unroll
+    for (int u = 0; u < XR; ++u) xr[u] = x[min(tid + u * nt, cols - 1)];
     float m = 0.f;
-    for (int c = threadIdx.x; c < cols; c += blockDim.x) m = fmaxf(m, fabsf(x[c]));
+#pragma unroll
+    for (int u = 0; u < XR; ++u)
+        if (tid + u * nt < cols) m = fmaxf(m, fabsf(xr[u]));
+    for (int c = tid + XR * nt; c < cols; c += nt) m = fmaxf(m, fabsf(x[c]));
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
     if (lane_id() == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     float s = red[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) s = fmaxf(s, red[w]);
-    for (int c = threadIdx.x; c < cols; c += blockDim.x) {
+    for (int c = tid, it = 0; c < cols; c += nt, ++it) {
         int q = 0;
         if (s > 0.f) {
-            float t = __fdiv_rn(x[c], s);
-            float u = __fmul_rn(127.0f, t);
-            q = (int)__builtin_rintf(u);
+            float xc = 0.f;
+#pragma unroll
+            for (int u = 0; u < XR; ++u)
+                if (it == u) xc = xr[u];
+            if (it >= XR) xc = x[c];
+            float t = __fdiv_rn(xc, s);
+            float y = __fmul_rn(127.0f, t);
+            q = (int)__builtin_rintf(y);
             q = q > 127 ? 127 : (q < -128 ? -128 : q);
         }
         qx[c] = (int8_t)q;
