@@ -270,6 +270,33 @@ def test_lowbit_batch_heavy_ties(acc, gpu, mode):
 
 
 @BATCH_MODES
+def test_lowbit_batch_all_rows_tie(acc, gpu, mode):
+    """Every row holds the same entries, so every score ties: every slice list
+    reaches the threshold (all slices rescanned, far more candidates than a
+    wavefront's register path holds), and the results are the lowest rows in
+    order."""
+    rows, cols = 5000, 64
+    ptr = np.arange(rows + 1, dtype=np.int64) * 8
+    idx = np.tile(np.arange(0, 64, 8, dtype=np.int32), rows)
+    val = np.full(rows * 8, 0.5, np.float32)
+    csr = acc.CSRMatrix.from_arrays(rows, cols, ptr, idx, val)
+    g = acc.GpuMatrix(csr, csr.pack(bits=8, split=8))
+    g.set_batch_mode(mode)
+    X = np.ones((12, cols), np.float32)
+    ks = np.array([1, 8, 16, 32, 50, 100] * 2, np.int32)
+    check_batch(acc, g, csr, 8, False, X, ks, 256, 8)  # M = 32 > the 10 slices of a partition
+    out = empty(acc, 12 * 100 * 2, np.int32)
+    g.topk_lowbit_batch(dev(acc, X), ks, 256, out, 100)
+    acc.synchronize()
+    a = out.numpy().reshape(12, 100, 2)
+    # each partition (625 rows) keeps its 32 lowest rows; on equal scores the
+    # lower row ranks first
+    expect = np.sort(np.concatenate([p * 625 + np.arange(32) for p in range(8)]))
+    for qi, k in enumerate(ks):
+        np.testing.assert_array_equal(a[qi, :k, 1], expect[:k])
+
+
+@BATCH_MODES
 def test_lowbit_batch_tiny_partitions(acc, gpu, mode):
     """Partitions of one or two slices (at most M slices: every slice is
     scanned), empty rows, K larger than some partitions hold."""
