@@ -276,7 +276,7 @@ int launch_lowbit_stage(acc_gpu_matrix *m, const LowbitArgs &a, int stage, hipSt
     } else if (stage == 1) {
         lowbit_fn ksel = get_lowbit_select(m->entry_bits, m->is_signed != 0);
         if (!ksel) return fail(ACC_ERR_ARG, "unsupported entry width");
-        const size_t lds_sel = 64 + 8 * 4096 + qlds;              // key list, query
+        const size_t lds_sel = 64 + 8 * 4096 + 5 * qlds;         // key list, int8 and FP32 query
         int rc = lds_limit((const void *)ksel, lds_sel + 40960);  // plus about 36 KB static
         if (rc != ACC_OK) return rc;
         LAUNCH(m, 1, ksel, dim3(m->split), dim3(SEL_WG), lds_sel, st, a);

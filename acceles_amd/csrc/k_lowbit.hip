@@ -144,6 +144,7 @@ __global__ __launch_bounds__(SEL_WG) void lowbit_select(LowbitArgs a) {
     float *red = (float *)smem;
     uint64_t *keys = (uint64_t *)(smem + 64);  // KCAP keys; the slow path's collectors reuse it
     int8_t *qx = (int8_t *)(smem + 64 + 8 * KCAP);
+    float *xs = (float *)(smem + 64 + 8 * KCAP + ((a.cols + 15) & ~15));  // FP32 query (rescoring)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int p = blockIdx.x;
     const int64_t s0 = a.part_slice[p];
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(SEL_WG) void lowbit_select(LowbitArgs a) {
         s_kn = 0;
         s_tn = 0;
     }
-    quantize_query_lds(a.x, a.cols, qx, red);
+    quantize_query_lds(a.x, a.cols, qx, red, xs);
 
     // 1. threshold (unsigned order: score ^ 2^31; key 0 = not a full slice):
     //    up to 256 slices and M <= 32 by maximum extraction in wavefront 0,
@@ -395,12 +396,13 @@ __global__ __launch_bounds__(SEL_WG) void lowbit_select(LowbitArgs a) {
     }
     if (a.K <= 0) return;
 
-    // 4. FP32 rescoring of the partition's candidates (16 entries per lane in flight)
+    // 4. FP32 rescoring of the partition's candidates (16 entries per lane in
+    //    flight; the query from LDS)
     for (int base = 0; base < a.M; base += SEL_WG / a.G) {
         const int i = base + (int)threadIdx.x / a.G;
         const bool valid = i < mcount;
         const int32_t r = valid ? dkey_row(top[i]) : 0;
-        const float sc = group_dot_rt<16>(a.ptr, a.idx, a.val, a.x, r, valid, a.G);
+        const float sc = group_dot_rt<16>(a.ptr, a.idx, a.val, xs, r, valid, a.G);
         if ((threadIdx.x & (a.G - 1)) == 0 && i < a.M) a.resc[(size_t)p * a.M + i] = valid ? dkey_f32(sc, r) : 0ull;
     }
 }

@@ -37,7 +37,8 @@
 // (quantify_vector with B=8, symmetric; float32 division and product, half-even).
 // The first 4*blockDim.x values are read once, with loads issued together
 // (clamped indices, masked after), and kept in registers for both passes.
-__device__ inline void quantize_query_lds(const float *x, int cols, int8_t *qx, float *red) {
+// xs (optional): the FP32 query is also copied there (LDS, cols floats).
+__device__ inline void quantize_query_lds(const float *x, int cols, int8_t *qx, float *red, float *xs = nullptr) {
     constexpr int XR = 4;
     const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
     float xr[XR];
@@ -54,13 +55,14 @@ __device__ inline void quantize_query_lds(const float *x, int cols, int8_t *qx, 
     float s = red[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) s = fmaxf(s, red[w]);
     for (int c = tid, it = 0; c < cols; c += nt, ++it) {
+        float xc = 0.f;
+#pragma unroll
+        for (int u = 0; u < XR; ++u)
+            if (it == u) xc = xr[u];
+        if (it >= XR) xc = x[c];
+        if (xs) xs[c] = xc;
         int q = 0;
         if (s > 0.f) {
-            float xc = 0.f;
-#pragma unroll
-            for (int u = 0; u < XR; ++u)
-                if (it == u) xc = xr[u];
-            if (it >= XR) xc = x[c];
             float t = __fdiv_rn(xc, s);
             float y = __fmul_rn(127.0f, t);
             q = (int)__builtin_rintf(y);
