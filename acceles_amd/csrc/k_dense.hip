@@ -440,9 +440,10 @@ __device__ __forceinline__ void stage_query(float *xs, const float *xq, int cols
 // order (LDS); slot i of the candidate list is keys[i] or empty past nn.
 // Each group of G lanes rescores C candidates at once in the declared order
 // (strided fmaf, then the butterfly), so the scores equal those of every
-// other rescoring path; xs is the FP32 query, staged in LDS.
+// other rescoring path; xs is the FP32 query, staged in LDS; ranges
+// (optional, LDS): the candidates' CSR entry ranges, already loaded.
 __device__ __forceinline__ void emit_candidates(const DenseArgs &a, const uint64_t *keys, int nn, size_t base,
-                                                const float *xs) {
+                                                const float *xs, const uint2 *ranges) {
     const int lane = lane_id(), M = a.M;
     if (a.cand_rows_out)
         for (int i = lane; i < M; i += 64) {
@@ -465,7 +466,18 @@ __device__ __forceinline__ void emit_candidates(const DenseArgs &a, const uint64
             r[c] = valid[c] ? dkey_row(key) : 0;
         }
         float sc[C];
-        group_dot_rt_multi<C, 4>(a.ptr, a.idx, a.val, xs, r, valid, G, sc);
+        if (ranges) {
+            uint32_t lo[C], hi[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const uint2 rg = ranges[min(i0 + c * groups + grp, 63)];
+                lo[c] = valid[c] ? rg.x : 0u;
+                hi[c] = valid[c] ? rg.y : 0u;
+            }
+            group_dot_ranges_multi<C, 4>(a.idx, a.val, xs, lo, hi, G, sc);
+        } else {
+            group_dot_rt_multi<C, 4>(a.ptr, a.idx, a.val, xs, r, valid, G, sc);
+        }
         if ((lane & (G - 1)) == 0)
 #pragma unroll
             for (int c = 0; c < C; ++c) {
@@ -603,6 +615,13 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
             if (n <= 64) {
                 wave_sync();
                 const uint64_t k = lane < n ? bufB[lane] : 0ull;
+                // the candidates' CSR entry ranges, in flight while the ranks
+                // are counted (out set: the CSR is present)
+                uint2 rg = make_uint2(0u, 0u);
+                if (a.out) {
+                    const int32_t r = lane < n ? dkey_row(k) : 0;
+                    rg = make_uint2(a.ptr[r], a.ptr[r + 1]);
+                }
                 const uint32_t klo = (uint32_t)k, khi = (uint32_t)(k >> 32);
                 int rank = 0;
                 for (int j = 0; j < n; ++j) {
@@ -611,9 +630,13 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
                     rank += (kj > k || (kj == k && j < lane)) ? 1 : 0;
                 }
                 const int nn = n < M ? n : M;
-                if (lane < n && rank < nn) bufA[rank] = k;
+                uint2 *ranges = (uint2 *)(bufA + 64);
+                if (lane < n && rank < nn) {
+                    bufA[rank] = k;
+                    ranges[rank] = rg;
+                }
                 wave_sync();
-                emit_candidates(a, bufA, nn, base, xs);
+                emit_candidates(a, bufA, nn, base, xs, ranges);
                 return;
             }
         }
@@ -664,7 +687,7 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
         offer_lists<U>(a, cb, k4, nv, row, s0, b, ts, q, qx, have_qx);
     }
     cb.compact();
-    emit_candidates(a, cb.buf, cb.n, base, xs);
+    emit_candidates(a, cb.buf, cb.n, base, xs, nullptr);
 }
 
 // One workgroup per query: each wavefront sorts a quarter of the query's

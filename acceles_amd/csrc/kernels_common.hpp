@@ -207,23 +207,21 @@ __device__ __forceinline__ float group_dot_rt(const uint32_t *ptr, const int32_t
     return p;
 }
 
-// The same dot products for C rows at once (rows r[c] against x): their
+// The same dot products for C rows at once, given their entry ranges
+// [lo[c], hi[c]) (lo = hi for an empty or invalid row), against x: their
 // loads are issued together, so C rows cost the memory round trips of one.
 // Each row is summed in the order of group_dot_rt (entries g, g+G, ... with
 // fmaf, then the butterfly), so the results are the same floats.
 template <int C, int U = 4>
-__device__ __forceinline__ void group_dot_rt_multi(const uint32_t *ptr, const int32_t *idx, const float *val,
-                                                   const float *x, const int32_t (&r)[C],
-                                                   const bool (&valid)[C], int G, float (&out)[C]) {
+__device__ __forceinline__ void group_dot_ranges_multi(const int32_t *idx, const float *val, const float *x,
+                                                       const uint32_t (&lo)[C], const uint32_t (&hi)[C], int G,
+                                                       float (&out)[C]) {
     const int g = lane_id() & (G - 1);
     uint32_t b[C], e[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        // r[c] is a row of the matrix also for invalid candidates, so the
-        // loads are unconditional and issue together
-        const uint32_t p0 = ptr[r[c]], p1 = ptr[r[c] + 1];
-        b[c] = valid[c] ? p0 + g : 0u;
-        e[c] = valid[c] ? p1 : 0u;
+        b[c] = lo[c] + g;
+        e[c] = hi[c];
         out[c] = 0.f;
     }
     for (uint32_t o = 0;; o += U * G) {
@@ -258,6 +256,22 @@ __device__ __forceinline__ void group_dot_rt_multi(const uint32_t *ptr, const in
 #pragma unroll
     for (int c = 0; c < C; ++c)
         for (int s = G >> 1; s >= 1; s >>= 1) out[c] = out[c] + __shfl_xor(out[c], s);
+}
+
+// group_dot_ranges_multi for rows r[c] (r[c] is a row of the matrix also for
+// invalid candidates, so the bound loads are unconditional and issue together).
+template <int C, int U = 4>
+__device__ __forceinline__ void group_dot_rt_multi(const uint32_t *ptr, const int32_t *idx, const float *val,
+                                                   const float *x, const int32_t (&r)[C],
+                                                   const bool (&valid)[C], int G, float (&out)[C]) {
+    uint32_t lo[C], hi[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const uint32_t p0 = ptr[r[c]], p1 = ptr[r[c] + 1];
+        lo[c] = valid[c] ? p0 : 0u;
+        hi[c] = valid[c] ? p1 : 0u;
+    }
+    group_dot_ranges_multi<C, U>(idx, val, x, lo, hi, G, out);
 }
 
 
