@@ -245,6 +245,7 @@ def test_lowbit_batch_uneven_k(acc, gpu, mode):
     (20000, 500, (40, 100), 4, 32, False, 9),    # 4-bit, one 32-query block
     (30000, 1024, (0.5, 40.0), 8, 64, False, 33),  # power-law rows, 1024 columns
     (12000, 300, (5, 30), 4, 8, False, 300),     # 300 queries: three 128-query blocks, a ragged last one
+    (40000, 4096, (0, 300), 4, 16, False, 24),   # 4096 columns (4-bit: scores stay below 2^25 for the tiles): query columns past what the kernels hold in registers
     (50000, 200, (10, 30), 8, 2, False, 20),     # partitions of 391 slices (threshold by extraction)
     (150000, 200, (10, 30), 8, 2, False, 20),    # partitions of 1172 slices (threshold by collectors)
 ])
