@@ -24,7 +24,14 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
                                                    const float *x, int rows, int cols, float *y) {
     extern __shared__ __align__(16) unsigned char smem[];
     float *xs = (float *)smem;
-    for (int c = threadIdx.x; c < cols; c += ACC_WG) xs[c] = x[c];
+    for (int c0 = 0; c0 < cols; c0 += 4 * ACC_WG) {  // four loads per thread issued together
+        float t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) t[u] = x[min(c0 + u * ACC_WG + (int)threadIdx.x, cols - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (c0 + u * ACC_WG + (int)threadIdx.x < cols) xs[c0 + u * ACC_WG + threadIdx.x] = t[u];
+    }
     __syncthreads();
     constexpr int RPI = 64 / G;  // rows per block
     constexpr int U = 4;
@@ -34,13 +41,18 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
     const int per = (nblocks + nw - 1) / nw;
     const int b0 = gw * per, b1 = min(nblocks, b0 + per);
     if (b0 >= b1) return;
+    // row bounds are loaded unconditionally (row clamped into the matrix) and
+    // masked after: a conditional load would be waited for before the step's
+    // entry loads are issued
     uint32_t jb[B], je[B];
 #pragma unroll
     for (int t = 0; t < B; ++t) {
         const int r = (b0 + t) * RPI + lane / G;
         const bool ok = b0 + t < b1 && r < rows;
-        jb[t] = ok ? ptr[r] : 0u;
-        je[t] = ok ? ptr[r + 1] : 0u;
+        const int rc = min(r, rows - 1);
+        const uint32_t p0 = ptr[rc], p1 = ptr[rc + 1];
+        jb[t] = ok ? p0 : 0u;
+        je[t] = ok ? p1 : 0u;
     }
     for (int b = b0; b < b1; b += B) {
         uint32_t nb[B], ne[B];
@@ -48,8 +60,10 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
         for (int t = 0; t < B; ++t) {  // the next step's row bounds
             const int r = (b + B + t) * RPI + lane / G;
             const bool ok = b + B + t < b1 && r < rows;
-            nb[t] = ok ? ptr[r] : 0u;
-            ne[t] = ok ? ptr[r + 1] : 0u;
+            const int rc = min(r, rows - 1);
+            const uint32_t p0 = ptr[rc], p1 = ptr[rc + 1];
+            nb[t] = ok ? p0 : 0u;
+            ne[t] = ok ? p1 : 0u;
         }
         float v[B][U];
         int c[B][U];
