@@ -488,10 +488,11 @@ __device__ __forceinline__ void emit_candidates(const DenseArgs &a, const uint64
 }
 
 // One wavefront per (partition p = blockIdx.x, query q = blockIdx.y*4 + wave):
-//  1. ts = the M-th best slice maximum of the partition (INT32_MIN when the
-//     partition has fewer than M full slices).  M distinct rows score >= ts,
-//     so the partition's M-th best score is >= ts and no row below ts can be
-//     in its Top-M.
+//  1. ts = a score that M distinct rows of the partition reach: the M-th best
+//     of the lanes' largest full-slice maxima on the register path, the M-th
+//     best full-slice maximum otherwise (INT32_MIN when there are not M).  The
+//     partition's M-th best score is >= ts, so no row below ts can be in its
+//     Top-M.
 //  2. Every listed key of a real row scoring >= ts is a candidate.  A list
 //     whose fourth key still scores >= ts may have dropped a qualifying row:
 //     that slice is rescanned exactly (all 64 rows, one dot4 pass) instead.
@@ -556,7 +557,12 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
             uint32_t v[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) v[u] = nv[u] == 64 ? k4[u].x : 0u;
-            const uint32_t t = wave_kth_u32<4>(v, M);
+            // the M-th largest of the lanes' largest full-slice maxima: M lanes
+            // hold a maximum >= it, i.e. M distinct rows score >= it -- a
+            // valid threshold, found with one 64-key register sort
+            uint32_t h[1] = {max(max(v[0], v[1]), max(v[2], v[3]))};
+            reg_sort_desc_u32<1>(h);
+            const uint32_t t = (uint32_t)__shfl((int)h[0], M - 1);
             if (t != 0u) ts = tk_score(t);
         }
         bool rescan[4], pass[16];
