@@ -210,11 +210,11 @@ def bench_large_single(a, acc, stream):
 
 def bench_batched(a, acc, stream):
     """2,000,000 x 900, row nnz U[30,90], L2-normalised; 256 queries per step with K drawn from
-    {1,8,16,32,50,75,100}; CK=512, SPLIT=32; autotuned layout; one HIP-graph launch per step."""
+    {1,8,16,32,50,75,100}; CK=512, SPLIT from --split (default 32); autotuned layout; one HIP-graph launch per step."""
     rows, cols, B, CK = 2_000_000, 900, 256, 512
     t0 = time.perf_counter()
     csr = acc.CSRMatrix.generate(rows, cols, deg=(30, 90), seed=0xACCE1E5 + 4)
-    pk = csr.pack(bits=a.bits, split=32)
+    pk = csr.pack(bits=a.bits, split=a.split)
     g = acc.GpuMatrix(csr, pk, device=0)
     log(f"[bench] setup {time.perf_counter() - t0:.1f}s: nnz={csr.nnz} packed={pk.info.stream_bytes / 1e6:.0f}MB")
     pts = g.autotune(CK, 100, B)
@@ -254,9 +254,9 @@ def bench_batched(a, acc, stream):
     line = line_common(a, "batched low-bit Top-K queries/s (mixed K)", qps, "queries/s", dt * 1e3 / a.steps,
                        "u8" if a.bits == 8 else f"u{a.bits}",
                        f"low-bit two-stage, batched: {rows}x{cols}, row nnz U[30,90], L2-normalised, {a.bits}-bit, "
-                       f"SPLIT=32, CK=512, {B} queries per HIP-graph launch, K from {{1,8,16,32,50,75,100}}, "
+                       f"SPLIT={a.split}, CK=512, {B} queries per HIP-graph launch, K from {{1,8,16,32,50,75,100}}, "
                        f"{NF} launches in flight (a plan per stream)",
-                       {"rows": rows, "cols": cols, "nnz": csr.nnz, "queries_per_step": B})
+                       {"rows": rows, "cols": cols, "nnz": csr.nnz, "split": a.split, "queries_per_step": B})
     line.update({"roofline": None, "cpu_baseline": None, "mean_precision_at_k": round(float(np.mean(prec)), 5),
                  "one_at_a_time_qps": round(a.steps * B / dt1, 2), "in_flight_equals_one_at_a_time": same,
                  "autotune": [{"nq": p.nq, "layout": int(p.layout), "us_single": round(p.us_single, 1),
