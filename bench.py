@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--inflight", type=int, default=0, choices=[0, 1, 2, 3, 4],
                     help="batches (embedding, batched) or queries (large-single) in flight on as many streams; "
-                         "0: 3 batches, 2 queries")
+                         "0: 3 batches (embedding), 2 batches (batched), 2 queries (large-single)")
     ap.add_argument("--batch-mode", choices=["auto", "packets", "dense"], default="auto",
                     help="scoring layout of the batched path")
     ap.add_argument("--workload", choices=["embedding", "large-single", "batched", "serve", "copy", "resparse"],
@@ -228,8 +228,9 @@ def bench_batched(a, acc, stream):
     plan = acc.Plan(g, acc.PLAN_LOWBIT_BATCH, Xd, ks, CK, out, 100)
     dt1 = timed(stream, lambda: plan.launch(stream), a.steps, a.warmup)
     res = out.numpy()
-    # batches in flight: one plan (own workspace) per stream
-    NF = a.inflight or 3
+    # batches in flight: one plan (own workspace) per stream; two beat three and four at this size
+    # (a 256-query tile pass keeps the chip busy; a third launch only contends for it)
+    NF = a.inflight or 2
     streams = [stream] + [acc.Stream(0) for _ in range(1, NF)]
     xs2 = [Xd] + [acc.DeviceBuffer.from_numpy(acc.generate_queries(B, cols, seed=0xACCE1E5 + 60 + j))
                   for j in range(1, NF)]
