@@ -273,8 +273,16 @@ def bench_serve(a, acc, stream):
     specs = [(1_000_000, 500, 60), (5_000_000, 900, 40), (10_000_000, 1024, 20)]
     t0 = time.perf_counter()
     srv = acc.Server(max_batch=256, slots=3, ck=512, autotune=True)
+    # probe queries per matrix with their FP64 exact Top-100 (scored on the GPU before the matrix is handed over)
+    NP = 8
+    probes = []
     for i, (r, c, d) in enumerate(specs):
         csr = acc.CSRMatrix.generate(r, c, deg=(d // 2, d + d // 2), l2norm=False, seed=0xACCE1E5 + 10 + i)
+        X = acc.generate_queries(NP, c, seed=0xACCE1E5 + 20 + i)
+        Xd = acc.DeviceBuffer.from_numpy(X)
+        g_exact = acc.GpuMatrix(csr, None)
+        probes.append((X, [exact_topk(g_exact, acc, Xd.at(j * c), r, 100, stream) for j in range(NP)]))
+        del g_exact, Xd
         srv.add_matrix(csr, csr.pack(bits=8, split=32))
         del csr
     log(f"[bench] serve setup {time.perf_counter() - t0:.1f}s, {srv.stats()['device_bytes'] / 1e9:.2f} GB resident")
@@ -284,13 +292,24 @@ def bench_serve(a, acc, stream):
         runs[str(rate)] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}
         log(f"[bench] serve {rate} q/s offered: {st['qps']:.0f} q/s done, p50 {st['p50_us']:.0f}us, "
             f"p99 {st['p99_us']:.0f}us, mean batch {st['mean_batch']:.1f}")
+    # precision@K of the served results against the FP64 exact Top-K; recall@K = share of probes with all K found
+    prec = []
+    for mid, (X, exact) in enumerate(probes):
+        tickets = [srv.submit(mid, X[j], 100) for j in range(len(X))]
+        for j, t in enumerate(tickets):
+            prec.append(precision_at(srv.wait(t, 100)["index"], exact[j]))
+    prec = np.array(prec)
+    precision = dict(zip(map(str, KS), np.round(prec.mean(axis=0), 5).tolist()))
+    recall = dict(zip(map(str, KS), np.round((prec == 1.0).mean(axis=0), 5).tolist()))
+    log(f"[bench] serve precision@K {precision}; recall@K (all K found) {recall} over {len(prec)} probes")
     srv.close()
     top = runs["50000"]
     line = line_common(a, "serving queries/s with p50/p99 latency (continuous batching)", top["qps"], "queries/s",
                        1e3 / max(top["qps"], 1e-9), "u8",
                        "serving: 1,000,000x500 deg 60 + 5,000,000x900 deg 40 + 10,000,000x1024 deg 20, 8-bit, CK=512, "
                        "Poisson arrivals, random matrix, K uniform in [1,100]", {"offered_qps": [2000, 10000, 50000]})
-    line.update({"roofline": None, "cpu_baseline": None, "runs": runs})
+    line.update({"roofline": None, "cpu_baseline": None, "runs": runs, "precision_at_k": precision,
+                 "recall_all_found_at_k": recall})
     print(json.dumps(line), flush=True)
 
 
