@@ -793,3 +793,4 @@ class GpuMatrix:
 def top_k_heap(y, n, k, out, stream=None):
     """top_k_heap (cpu_mkl_baseline.cpp:128-149) on a device score vector."""
     _check(lib().acc_gpu_top_k_heap(_ptr(y), n, k, _ptr(out), _stream(stream)))
+

@@ -56,6 +56,12 @@ struct BatchWs {
     size_t y_n = 0;
     uint64_t *rsel = nullptr;    // FP32 plans: radix selection keys
     size_t rsel_n = 0;
+    uint64_t *sthr = nullptr;    // FP32 plans, K > 256: select_blocks state (zero between launches)
+    size_t sthr_n = 0;
+    uint32_t *scnt = nullptr;
+    size_t scnt_n = 0;
+    uint64_t *scand = nullptr;
+    size_t scand_n = 0;
 };
 
 struct acc_gpu_matrix {
@@ -1204,12 +1210,24 @@ static int plan_enqueue(acc_plan *p, bool upload_k, hipStream_t st) {
             return run_batch(m, p->ws, p->x, p->nq, p->k.data(), upload_k, p->kmax, p->ck, p->out, p->out_stride, st);
         case ACC_PLAN_FP32: {
             const size_t rcap = radix_select_keys(m->rows, 256);
-            int rc = ws_grow(p->ws, &p->ws.y, &p->ws.y_n, (size_t)m->rows);
-            if (rc == ACC_OK) rc = ws_grow(p->ws, &p->ws.rsel, &p->ws.rsel_n, rcap);
-            if (rc == ACC_OK) rc = launch_spmv_fp32(m, p->x, p->ws.y, st);
+            BatchWs &w = p->ws;
+            const bool fresh = w.sthr == nullptr;
+            int rc = ws_grow(w, &w.y, &w.y_n, (size_t)m->rows);
+            if (rc == ACC_OK) rc = ws_grow(w, &w.rsel, &w.rsel_n, rcap);
+            // the plan's own select_blocks state, so that plans of one handle
+            // can run concurrently for any K; zeroed once by a kernel (the
+            // kernels leave it zero), before capture
+            if (rc == ACC_OK) rc = ws_grow(w, &w.sthr, &w.sthr_n, 1);
+            if (rc == ACC_OK) rc = ws_grow(w, &w.scnt, &w.scnt_n, 1);
+            if (rc == ACC_OK) rc = ws_grow(w, &w.scand, &w.scand_n, (size_t)m->sel_blocks * kMaxK);
+            if (rc == ACC_OK && fresh) {
+                hipLaunchKernelGGL(get_select_reset(), dim3(1), dim3(64), 0, st, w.sthr, w.scnt);
+                rc = check_launch();
+            }
+            if (rc == ACC_OK) rc = launch_spmv_fp32(m, p->x, w.y, st);
             if (rc != ACC_OK) return rc;
-            return launch_select(p->ws.y, m->rows, p->kmax, m->sel_blocks, m->sel_chunk, m->d_sthr, m->d_scnt,
-                                 m->d_scand, p->out, st, nullptr, p->ws.rsel, rcap);
+            return launch_select(w.y, m->rows, p->kmax, m->sel_blocks, m->sel_chunk, w.sthr, w.scnt, w.scand, p->out,
+                                 st, nullptr, w.rsel, rcap);
         }
         default:
             return fail(ACC_ERR_ARG, "unknown plan kind");
@@ -1314,45 +1332,58 @@ extern "C" int acc_topk_lowbit_host(acc_gpu_matrix *m, const float *x_host, int 
     return ACC_OK;
 }
 
+// The reference's src/cpu query loop: query t is U(-1, 1) from the seed
+// 0x5EED + t (generate_random_vector, cpu_mkl_baseline.cpp:111-122, is
+// unseeded there), the SpMV and the Top-k are timed separately with events
+// on the null stream, and the last query's results are returned.  Everything
+// it touches on the device belongs to the handle (scores, selection state,
+// staging), so a call allocates nothing; every runtime status is checked.
 extern "C" int acc_top_k_spmv(acc_gpu_matrix *A, int row, int col, int k, int num_threads, acc_element *top_k,
                               int num_test) {
     (void)num_threads;
     if (!A || !top_k || num_test < 1) return fail(ACC_ERR_ARG, "bad arguments");
+    if (!A->has_csr) return fail(ACC_ERR_ARG, "matrix was uploaded without the FP32 CSR");
     if (row != A->rows || col != A->cols) return fail(ACC_ERR_ARG, "row/col do not match the matrix");
     if (k < 1 || k > kMaxK) return fail(ACC_ERR_ARG, "k must be in [1, 960]");
     HIP_TRY(hipSetDevice(A->device));
     std::vector<float> x((size_t)col);
-    float *d_y = nullptr;
-    HIP_TRY(hipMalloc((void **)&d_y, (size_t)row * 4 + 4));
-    hipEvent_t e0, e1, e2;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventCreate(&e2);
-    double cpt = 0, srt = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     int rc = ACC_OK;
+    for (auto &e : ev)
+        if (rc == ACC_OK && hipEventCreate(&e) != hipSuccess) rc = fail(ACC_ERR_DEVICE, "hipEventCreate failed");
+    double cpt = 0, srt = 0;
     for (int t = 0; t < num_test && rc == ACC_OK; ++t) {
-        acc_generate_queries(1, col, ACC_VAL_UNIFORM, 0, 0x5EEDull + (uint64_t)t, x.data());
-        hipMemcpy(A->d_xstage, x.data(), (size_t)col * 4, hipMemcpyHostToDevice);
-        hipEventRecord(e0, nullptr);
-        rc = acc_gpu_spmv_fp32(A, A->d_xstage, d_y, nullptr);
-        hipEventRecord(e1, nullptr);
-        if (rc == ACC_OK) rc = acc_gpu_top_k_heap(d_y, row, k, A->d_ostage, nullptr);
-        hipEventRecord(e2, nullptr);
-        hipEventSynchronize(e2);
-        float a = 0, b = 0;
-        hipEventElapsedTime(&a, e0, e1);
-        hipEventElapsedTime(&b, e1, e2);
+        rc = acc_generate_queries(1, col, ACC_VAL_UNIFORM, 0, 0x5EEDull + (uint64_t)t, x.data());
+        if (rc != ACC_OK) break;
+        if (hipMemcpy(A->d_xstage, x.data(), (size_t)col * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipEventRecord(ev[0], nullptr) != hipSuccess) {
+            rc = fail(ACC_ERR_DEVICE, "query upload failed");
+            break;
+        }
+        rc = launch_spmv_fp32(A, A->d_xstage, A->d_y, nullptr);
+        if (rc == ACC_OK && hipEventRecord(ev[1], nullptr) != hipSuccess) rc = fail(ACC_ERR_DEVICE, "hipEventRecord failed");
+        if (rc == ACC_OK)
+            rc = launch_select(A->d_y, A->rows, k, A->sel_blocks, A->sel_chunk, A->d_sthr, A->d_scnt, A->d_scand,
+                               A->d_ostage, nullptr, nullptr, A->d_rsel, A->rsel_cap);
+        if (rc != ACC_OK) break;
+        float a = 0.f, b = 0.f;
+        if (hipEventRecord(ev[2], nullptr) != hipSuccess || hipEventSynchronize(ev[2]) != hipSuccess ||
+            hipEventElapsedTime(&a, ev[0], ev[1]) != hipSuccess || hipEventElapsedTime(&b, ev[1], ev[2]) != hipSuccess) {
+            rc = fail(ACC_ERR_DEVICE, "query timing failed");
+            break;
+        }
         cpt += a;
         srt += b;
     }
     if (rc == ACC_OK && hipMemcpy(top_k, A->d_ostage, (size_t)k * sizeof(acc_element), hipMemcpyDeviceToHost) != hipSuccess)
         rc = fail(ACC_ERR_DEVICE, "copy of results failed");
-    std::printf("compute time: %g ms\n", cpt / num_test);
-    std::printf("sort time: %g ms\n", srt / num_test);
-    std::printf("exec time: %gms\n", (cpt + srt) / num_test);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipEventDestroy(e2);
-    hipFree(d_y);
+    if (rc == ACC_OK) {
+        std::printf("compute time: %g ms\n", cpt / num_test);
+        std::printf("sort time: %g ms\n", srt / num_test);
+        std::printf("exec time: %gms\n", (cpt + srt) / num_test);
+        std::fflush(stdout);
+    }
+    for (auto &e : ev)
+        if (e) hipEventDestroy(e);
     return rc;
 }

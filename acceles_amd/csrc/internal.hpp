@@ -4,6 +4,7 @@
 #include "../../include/acceles.h"
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -59,5 +60,47 @@ inline uint32_t ordered_f32(float f) {
 }
 
 int bits_for(int32_t n);  // ceil(log2(n)), at least 1
+
+// Device-memory admission: byte reservations against a fixed limit, each
+// checked and taken under one lock, so concurrent callers cannot both pass
+// against the same room.
+class Budget {
+  public:
+    void set_limit(int64_t limit) {
+        std::lock_guard<std::mutex> l(mu_);
+        limit_ = limit;
+    }
+    // Reserves n bytes if they fit.
+    bool reserve(int64_t n) {
+        std::lock_guard<std::mutex> l(mu_);
+        if (n < 0 || used_ + n > limit_) return false;
+        used_ += n;
+        return true;
+    }
+    // Replaces a reservation of `from` bytes by one of `to` bytes if the
+    // result fits (shrinking always does).
+    bool resize(int64_t from, int64_t to) {
+        std::lock_guard<std::mutex> l(mu_);
+        if (to > from && used_ - from + to > limit_) return false;
+        used_ += to - from;
+        return true;
+    }
+    void release(int64_t n) {
+        std::lock_guard<std::mutex> l(mu_);
+        used_ -= n;
+    }
+    int64_t used() const {
+        std::lock_guard<std::mutex> l(mu_);
+        return used_;
+    }
+    int64_t limit() const {
+        std::lock_guard<std::mutex> l(mu_);
+        return limit_;
+    }
+
+  private:
+    mutable std::mutex mu_;
+    int64_t limit_ = 0, used_ = 0;
+};
 
 }  // namespace acc

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 
 using acc::fail;
 
@@ -349,12 +350,128 @@ bool put_vec(FILE *f, const std::vector<T> &v) {
     const uint64_t n = v.size();
     return put(f, n) && (n == 0 || std::fwrite(v.data(), sizeof(T), n, f) == n);
 }
+// Bytes from the current position to the end of the file.
+uint64_t bytes_left(FILE *f) {
+    const long pos = std::ftell(f);
+    if (pos < 0 || std::fseek(f, 0, SEEK_END) != 0) return 0;
+    const long end = std::ftell(f);
+    std::fseek(f, pos, SEEK_SET);
+    return end > pos ? (uint64_t)(end - pos) : 0;
+}
+
+// A vector of at most max_n elements; a length field larger than what the
+// file still holds is refused before anything is allocated.
 template <typename T>
 bool get_vec(FILE *f, std::vector<T> &v, uint64_t max_n) {
     uint64_t n = 0;
-    if (!get(f, n) || n > max_n) return false;
+    if (!get(f, n) || n > max_n || n > bytes_left(f) / sizeof(T)) return false;
     v.resize(n);
     return n == 0 || std::fread(v.data(), sizeof(T), n, f) == n;
+}
+
+// What the kernels and the host index with, checked on a loaded packet file;
+// the entry statistics are recomputed from the decoded entries rather than
+// taken from the header.
+bool valid_packed(acc_packed *pk) {
+    const int S = pk->split;
+    const int64_t NS = pk->nslices();
+    if (pk->entry_bits != pk->col_bits + pk->bits || pk->entry_bits < 10 || pk->entry_bits > 22 || pk->bits < 2 ||
+        pk->bits > 8 || pk->cols < 1 || pk->cols > (1 << pk->col_bits) || pk->rows < 1)
+        return false;
+    if (pk->part_row_begin.size() != (size_t)S + 1 || pk->part_slice_begin.size() != (size_t)S + 1 ||
+        pk->slice_ndw.size() != (size_t)NS || pk->perm.size() != (size_t)NS * 64)
+        return false;
+    // partitions: row ranges from 0 to rows, slice ranges from 0 to NS, each
+    // partition ceil(rows_p / 64) slices
+    if (pk->part_row_begin[0] != 0 || pk->part_row_begin[(size_t)S] != pk->rows || pk->part_slice_begin[0] != 0 ||
+        pk->part_slice_begin[(size_t)S] != NS)
+        return false;
+    for (int p = 0; p < S; ++p) {
+        const int64_t nr = (int64_t)pk->part_row_begin[(size_t)p + 1] - pk->part_row_begin[(size_t)p];
+        const int64_t ns = pk->part_slice_begin[(size_t)p + 1] - pk->part_slice_begin[(size_t)p];
+        if (nr < 0 || ns != (nr + 63) / 64) return false;
+    }
+    for (int64_t s = 0; s < NS; ++s)
+        if (((uint64_t)pk->slice_off[(size_t)s] + pk->slice_ndw[(size_t)s]) * 64 > pk->words.size()) return false;
+    // lanes: padding (-1) only closes a partition's last slice, and the
+    // partition's real lanes name each of its rows_p rows; over the whole
+    // matrix every row appears exactly once
+    std::vector<uint8_t> seen((size_t)pk->rows, 0);
+    for (int p = 0; p < S; ++p) {
+        const int64_t nr = (int64_t)pk->part_row_begin[(size_t)p + 1] - pk->part_row_begin[(size_t)p];
+        const int64_t base = pk->part_slice_begin[(size_t)p] * 64;
+        const int64_t end = pk->part_slice_begin[(size_t)p + 1] * 64;
+        for (int64_t i = base; i < end; ++i) {
+            const int32_t r = pk->perm[(size_t)i];
+            if (i - base < nr) {
+                if (r < 0 || r >= pk->rows || seen[(size_t)r]) return false;
+                seen[(size_t)r] = 1;
+            } else if (r != -1) {
+                return false;
+            }
+        }
+    }
+    // entries: every non-zero value names a column below cols; recount the
+    // kept entries, the rows naming a column twice and the range of a row's
+    // per-column sums (the dense layout's eligibility depends on them)
+    const int E = pk->entry_bits, cb = pk->col_bits, vb = pk->bits;
+    const uint32_t cmask = (1u << cb) - 1u, vmask = (1u << vb) - 1u;
+    bool ok = true;
+    int64_t kept = 0, dups = 0;
+    int32_t mmax = 0, mmin = 0;
+#pragma omp parallel reduction(&& : ok) reduction(+ : kept, dups) reduction(max : mmax) reduction(min : mmin)
+    {
+        std::vector<int32_t> sum((size_t)pk->cols, 0);
+        std::vector<uint32_t> mark((size_t)pk->cols / 32 + 1, 0u);
+        std::vector<int32_t> used;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t s = 0; s < NS; ++s) {
+            const uint32_t *w = pk->words.data() + (size_t)pk->slice_off[(size_t)s] * 64;
+            const int64_t ndw = pk->slice_ndw[(size_t)s], nent = ndw * 32 / E;
+            for (int l = 0; l < 64 && ok; ++l) {
+                bool dup = false;
+                used.clear();
+                for (int64_t j = 0; j < nent; ++j) {
+                    const int64_t pos = j * E, wi = pos >> 5;
+                    const int sh = (int)(pos & 31);
+                    const uint64_t lo = w[(size_t)wi * 64 + l];
+                    const uint64_t hi = wi + 1 < ndw ? w[(size_t)(wi + 1) * 64 + l] : 0;
+                    const uint32_t x = (uint32_t)(((hi << 32) | lo) >> sh);
+                    const uint32_t v = (x >> cb) & vmask, c = x & cmask;
+                    if (v == 0) continue;
+                    if ((int64_t)c >= pk->cols) {
+                        ok = false;
+                        break;
+                    }
+                    int32_t sv = (int32_t)v;
+                    if (pk->is_signed && (v >> (vb - 1))) sv = (int32_t)(v | ~vmask);
+                    if (pk->perm[(size_t)s * 64 + l] < 0) continue;
+                    ++kept;
+                    uint32_t &mk = mark[c >> 5];
+                    if (mk & (1u << (c & 31))) {
+                        dup = true;
+                    } else {
+                        mk |= 1u << (c & 31);
+                        used.push_back((int32_t)c);
+                    }
+                    sum[c] += sv;
+                }
+                for (int32_t c : used) {
+                    mmax = std::max(mmax, sum[(size_t)c]);
+                    mmin = std::min(mmin, sum[(size_t)c]);
+                    sum[(size_t)c] = 0;
+                    mark[(size_t)c >> 5] = 0u;
+                }
+                dups += dup;
+            }
+        }
+    }
+    if (!ok) return false;
+    pk->nnz_kept = kept;
+    pk->dup_rows = dups;
+    pk->merged_max = mmax;
+    pk->merged_min = mmin;
+    return true;
 }
 
 }  // namespace
@@ -370,24 +487,28 @@ extern "C" int acc_csr_save(const acc_csr *m, const char *path) {
 
 extern "C" int acc_csr_load(const char *path, acc_csr **out) {
     if (!path || !out) return fail(ACC_ERR_ARG, "null argument");
-    File r(path, "rb");
-    if (!r.f) return fail(ACC_ERR_IO, std::string("cannot open ") + path);
-    char magic[8];
-    acc_csr *m = new acc_csr();
-    bool ok = std::fread(magic, 1, 8, r.f) == 8 && std::memcmp(magic, kCsrMagic, 8) == 0 && get(r.f, m->rows) &&
-              get(r.f, m->cols) && m->rows >= 0 && m->cols >= 0 && get_vec(r.f, m->ptr, (uint64_t)m->rows + 1) &&
-              m->ptr.size() == (size_t)m->rows + 1 && get_vec(r.f, m->idx, UINT64_MAX / 8) &&
-              get_vec(r.f, m->val, m->idx.size());
-    // the arrays must describe a CSR matrix: monotone ptr, in-range columns
-    if (ok) ok = m->ptr[0] == 0 && m->ptr[(size_t)m->rows] == (int64_t)m->idx.size() && m->val.size() == m->idx.size();
-    for (int32_t i = 0; ok && i < m->rows; ++i) ok = m->ptr[(size_t)i] <= m->ptr[(size_t)i + 1];
-    for (size_t j = 0; ok && j < m->idx.size(); ++j) ok = m->idx[j] >= 0 && m->idx[j] < m->cols;
-    if (!ok) {
-        delete m;
-        return fail(ACC_ERR_FORMAT, std::string("not a CSR file of this library: ") + path);
+    try {
+        File r(path, "rb");
+        if (!r.f) return fail(ACC_ERR_IO, std::string("cannot open ") + path);
+        char magic[8];
+        acc_csr *m = new acc_csr();
+        bool ok = std::fread(magic, 1, 8, r.f) == 8 && std::memcmp(magic, kCsrMagic, 8) == 0 && get(r.f, m->rows) &&
+                  get(r.f, m->cols) && m->rows >= 0 && m->cols >= 0 && get_vec(r.f, m->ptr, (uint64_t)m->rows + 1) &&
+                  m->ptr.size() == (size_t)m->rows + 1 && get_vec(r.f, m->idx, UINT64_MAX / 8) &&
+                  get_vec(r.f, m->val, m->idx.size());
+        // the arrays must describe a CSR matrix: monotone ptr, in-range columns
+        if (ok) ok = m->ptr[0] == 0 && m->ptr[(size_t)m->rows] == (int64_t)m->idx.size() && m->val.size() == m->idx.size();
+        for (int32_t i = 0; ok && i < m->rows; ++i) ok = m->ptr[(size_t)i] <= m->ptr[(size_t)i + 1];
+        for (size_t j = 0; ok && j < m->idx.size(); ++j) ok = m->idx[j] >= 0 && m->idx[j] < m->cols;
+        if (!ok) {
+            delete m;
+            return fail(ACC_ERR_FORMAT, std::string("not a CSR file of this library: ") + path);
+        }
+        *out = m;
+        return ACC_OK;
+    } catch (const std::exception &e) {
+        return fail(ACC_ERR_NOMEM, std::string("loading ") + path + ": " + e.what());
     }
-    *out = m;
-    return ACC_OK;
 }
 
 extern "C" int acc_packed_save(const acc_packed *pk, const char *path) {
@@ -405,32 +526,28 @@ extern "C" int acc_packed_save(const acc_packed *pk, const char *path) {
 
 extern "C" int acc_packed_load(const char *path, acc_packed **out) {
     if (!path || !out) return fail(ACC_ERR_ARG, "null argument");
-    File r(path, "rb");
-    if (!r.f) return fail(ACC_ERR_IO, std::string("cannot open ") + path);
-    char magic[8];
-    acc_packed *pk = new acc_packed();
-    bool ok = std::fread(magic, 1, 8, r.f) == 8 && std::memcmp(magic, kPackMagic, 8) == 0 && get(r.f, pk->rows) &&
-              get(r.f, pk->cols) && get(r.f, pk->split) && get(r.f, pk->bits) && get(r.f, pk->col_bits) &&
-              get(r.f, pk->entry_bits) && get(r.f, pk->is_signed) && get(r.f, pk->scale) && get(r.f, pk->nnz) &&
-              get(r.f, pk->nnz_kept) && get(r.f, pk->dup_rows) && get(r.f, pk->merged_max) &&
-              get(r.f, pk->merged_min) && pk->split >= 1 && pk->rows >= pk->split &&
-              get_vec(r.f, pk->part_row_begin, (uint64_t)pk->split + 1) &&
-              get_vec(r.f, pk->part_slice_begin, (uint64_t)pk->split + 1) && get_vec(r.f, pk->slice_off, UINT64_MAX / 8) &&
-              get_vec(r.f, pk->slice_ndw, pk->slice_off.size()) && get_vec(r.f, pk->perm, pk->slice_off.size() * 64) &&
-              get_vec(r.f, pk->words, UINT64_MAX / 8);
-    // consistency of what the kernels index with: slices inside the words, lanes in range, widths
-    const int64_t NS = (int64_t)pk->slice_off.size();
-    if (ok) ok = pk->part_row_begin.size() == (size_t)pk->split + 1 && pk->part_slice_begin.size() == (size_t)pk->split + 1 &&
-                 pk->part_slice_begin[(size_t)pk->split] == NS && pk->slice_ndw.size() == (size_t)NS &&
-                 pk->perm.size() == (size_t)NS * 64 && pk->entry_bits == pk->col_bits + pk->bits &&
-                 pk->entry_bits >= 10 && pk->entry_bits <= 22 && pk->bits >= 2 && pk->bits <= 8;
-    for (int64_t s = 0; ok && s < NS; ++s)
-        ok = ((uint64_t)pk->slice_off[(size_t)s] + pk->slice_ndw[(size_t)s]) * 64 <= pk->words.size();
-    for (size_t i = 0; ok && i < pk->perm.size(); ++i) ok = pk->perm[i] >= -1 && pk->perm[i] < pk->rows;
-    if (!ok) {
-        delete pk;
-        return fail(ACC_ERR_FORMAT, std::string("not a packet file of this library: ") + path);
+    try {
+        File r(path, "rb");
+        if (!r.f) return fail(ACC_ERR_IO, std::string("cannot open ") + path);
+        char magic[8];
+        acc_packed *pk = new acc_packed();
+        bool ok = std::fread(magic, 1, 8, r.f) == 8 && std::memcmp(magic, kPackMagic, 8) == 0 && get(r.f, pk->rows) &&
+                  get(r.f, pk->cols) && get(r.f, pk->split) && get(r.f, pk->bits) && get(r.f, pk->col_bits) &&
+                  get(r.f, pk->entry_bits) && get(r.f, pk->is_signed) && get(r.f, pk->scale) && get(r.f, pk->nnz) &&
+                  get(r.f, pk->nnz_kept) && get(r.f, pk->dup_rows) && get(r.f, pk->merged_max) &&
+                  get(r.f, pk->merged_min) && pk->split >= 1 && pk->rows >= pk->split && pk->col_bits >= 1 &&
+                  pk->col_bits <= 20 && get_vec(r.f, pk->part_row_begin, (uint64_t)pk->split + 1) &&
+                  get_vec(r.f, pk->part_slice_begin, (uint64_t)pk->split + 1) &&
+                  get_vec(r.f, pk->slice_off, UINT64_MAX / 8) && get_vec(r.f, pk->slice_ndw, pk->slice_off.size()) &&
+                  get_vec(r.f, pk->perm, pk->slice_off.size() * 64) && get_vec(r.f, pk->words, UINT64_MAX / 8);
+        if (ok) ok = valid_packed(pk);
+        if (!ok) {
+            delete pk;
+            return fail(ACC_ERR_FORMAT, std::string("not a packet file of this library: ") + path);
+        }
+        *out = pk;
+        return ACC_OK;
+    } catch (const std::exception &e) {
+        return fail(ACC_ERR_NOMEM, std::string("loading ") + path + ": " + e.what());
     }
-    *out = pk;
-    return ACC_OK;
 }

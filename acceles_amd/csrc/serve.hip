@@ -85,7 +85,7 @@ struct acc_server {
     std::thread worker;
     bool stop = false;
     uint64_t next_id = 1;
-    int64_t used = 0;
+    acc::Budget mem;  // device bytes: slot buffers and resident matrices with their workspaces
     int max_cols = 8192;
     int64_t npending = 0;
     // statistics since the last reset
@@ -164,11 +164,16 @@ void launch(acc_server *s) {
             rc = ACC_ERR_DEVICE;
         if (rc == ACC_OK && hipEventRecord(sl.ev, sl.st) != hipSuccess) rc = ACC_ERR_DEVICE;
         if (rc != ACC_OK) {  // fail these requests; the slot stays free
+            const auto now = Clock::now();
             for (uint64_t id : sl.ids) {
-                Request &r = s->reqs[id];
+                auto it = s->reqs.find(id);
+                if (it == s->reqs.end()) continue;
+                Request &r = it->second;
                 r.done = true;
                 r.status = rc;
+                r.t1 = now;
                 ++s->completed;
+                if (!r.keep) s->reqs.erase(it);  // nobody waits for a load-generator request
             }
             sl.ids.clear();
             s->cv_done.notify_all();
@@ -233,19 +238,19 @@ extern "C" int acc_server_create(const acc_server_opts *opts, acc_server **out) 
     }
     acc_server *s = new acc_server();
     s->o = o;
+    s->mem.set_limit(o.memory_budget);
     int rc = ACC_OK;
+    const size_t xb = (size_t)o.max_batch * s->max_cols * 4, ob = (size_t)o.max_batch * kMaxK * sizeof(acc_element);
+    if (!s->mem.reserve((int64_t)(xb + ob) * o.slots)) rc = fail(ACC_ERR_BUDGET, "memory budget below the slot buffers");
     for (int i = 0; i < o.slots && rc == ACC_OK; ++i) {
         Slot &sl = s->slots[i];
-        const size_t xb = (size_t)o.max_batch * s->max_cols * 4, ob = (size_t)o.max_batch * kMaxK * sizeof(acc_element);
         if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||
             hipHostMalloc((void **)&sl.x_pin, xb, hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void **)&sl.o_pin, ob, hipHostMallocDefault) != hipSuccess ||
             hipMalloc((void **)&sl.x_dev, xb) != hipSuccess || hipMalloc((void **)&sl.o_dev, ob) != hipSuccess)
             rc = fail(ACC_ERR_NOMEM, "server slot allocation failed");
-        s->used += (int64_t)(xb + ob);
     }
-    if (rc == ACC_OK && s->used > o.memory_budget) rc = fail(ACC_ERR_BUDGET, "memory budget below the slot buffers");
     if (rc != ACC_OK) {
         free_slots(s);
         delete s;
@@ -256,25 +261,41 @@ extern "C" int acc_server_create(const acc_server_opts *opts, acc_server **out) 
     return ACC_OK;
 }
 
+// Device bytes acc_gpu_upload will hold for this matrix, before the
+// workspaces: the FP32 CSR (4-byte row pointers), the packets with their
+// padding, metadata and slot map, and per-row scores.
+static int64_t upload_estimate(const acc_csr *csr, const acc_packed *pk) {
+    const int64_t nnz = csr->nnz();
+    int64_t b = ((int64_t)csr->rows + 1) * 4 + nnz * 8 + (int64_t)csr->rows * 4;
+    b += (int64_t)pk->words.size() * 4 + (int64_t)pk->nslices() * (8 + 4 + 1 + 256) + 32 * 64 * 4;
+    return b;
+}
+
 extern "C" int acc_server_add_matrix(acc_server *s, const acc_csr *csr, const acc_packed *pk, int *matrix_id) {
     if (!s || !csr || !pk || !matrix_id) return fail(ACC_ERR_ARG, "a server, the FP32 CSR and its packets are required");
     if (csr->cols > s->max_cols) return fail(ACC_ERR_ARG, "cols must be <= 8192");
     if (hipSetDevice(s->o.device) != hipSuccess) return fail(ACC_ERR_DEVICE, "hipSetDevice failed");
+    // admission: the matrix's bytes are reserved under the lock before the
+    // upload, so concurrent additions cannot both pass against the same room;
+    // the reservation is corrected to the measured bytes afterwards
+    const int64_t reserved = upload_estimate(csr, pk);
+    if (!s->mem.reserve(reserved)) return fail(ACC_ERR_BUDGET, "matrix does not fit in the server's memory budget");
+    auto release = [&](int rc) {
+        s->mem.release(reserved);
+        return rc;
+    };
+    // the room this matrix may grow into: what is free plus its reservation
+    const int64_t have = s->mem.limit() - s->mem.used() + reserved;
     acc_gpu_matrix *m = nullptr;
     int rc = acc_gpu_upload(csr, pk, s->o.device, &m);
-    if (rc != ACC_OK) return rc;
-    // admission: the resident matrix plus the workspace of a full batch and of
-    // a small one (dense tiles, if the matrix can use them, are built here)
+    if (rc != ACC_OK) return release(rc);
+    // the resident matrix plus the workspace of a full batch and of a small
+    // one (dense tiles, if the matrix can use them, are built here)
     acc_gpu_info_t info;
     acc_gpu_matrix_info(m, &info);
-    int64_t have = 0;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        have = s->o.memory_budget - s->used;
-    }
     if (info.device_bytes > have) {
         acc_gpu_matrix_destroy(m);
-        return fail(ACC_ERR_BUDGET, "matrix does not fit in the server's memory budget");
+        return release(fail(ACC_ERR_BUDGET, "matrix does not fit in the server's memory budget"));
     }
     if (info.dense_ok) {
         const int64_t nsl = (info.rows + 63) / 64 + info.split;
@@ -299,17 +320,18 @@ extern "C" int acc_server_add_matrix(acc_server *s, const acc_csr *csr, const ac
     if (rc == ACC_OK && s->o.autotune)
         rc = acc_gpu_autotune(m, s->o.ck, std::min(100, std::min(kMaxK, s->o.ck)), s->o.max_batch, nullptr, 0, nullptr);
     if (rc == ACC_OK) acc_gpu_matrix_info(m, &info);
-    if (rc == ACC_OK && info.device_bytes > have) rc = fail(ACC_ERR_BUDGET, "matrix and its batch workspace exceed the budget");
+    // the reservation becomes the measured bytes, if they still fit
+    if (rc == ACC_OK && !s->mem.resize(reserved, info.device_bytes))
+        rc = fail(ACC_ERR_BUDGET, "matrix and its batch workspace exceed the budget");
     if (rc != ACC_OK) {
         acc_gpu_matrix_destroy(m);
-        return rc;
+        return release(rc);
     }
     std::lock_guard<std::mutex> lk(s->mu);
     Matrix mx;
     mx.m = m;
     mx.cols = csr->cols;
     mx.bytes = info.device_bytes;
-    s->used += mx.bytes;
     s->mats.push_back(std::move(mx));
     *matrix_id = (int)s->mats.size() - 1;
     return ACC_OK;
@@ -379,7 +401,7 @@ static void fill_stats(acc_server *s, acc_server_stats_t *st) {
     }
     const double secs = std::chrono::duration<double>(Clock::now() - s->t_reset).count();
     st->qps = secs > 0 ? (double)s->completed / secs : 0.0;
-    st->device_bytes = s->used;
+    st->device_bytes = s->mem.used();
 }
 
 extern "C" int acc_server_stats(acc_server *s, acc_server_stats_t *st) {

@@ -355,8 +355,10 @@ int acc_gpu_autotune(acc_gpu_matrix *m, int ck, int k, int max_batch, acc_tune_p
  *   ACC_PLAN_LOWBIT        acc_gpu_topk_lowbit, nq = 1, k = k_host[0];
  *   ACC_PLAN_LOWBIT_BATCH  acc_gpu_topk_lowbit_batch (per-query K);
  *   ACC_PLAN_FP32          acc_gpu_topk_fp32, nq = 1, k = k_host[0].
- * Launches of plans and calls on one handle must not overlap in time (they
- * share its selection state): issue them on one stream. */
+ * Every plan owns its query workspace and selection state, so different
+ * plans of one handle may run concurrently on different streams; launches of
+ * one plan, and the handle's own query calls (which share the handle's
+ * workspace), must not overlap in time. */
 typedef struct acc_plan acc_plan;
 #define ACC_PLAN_LOWBIT 1
 #define ACC_PLAN_LOWBIT_BATCH 2
@@ -425,6 +427,7 @@ int acc_gpu_candidates_lowbit(acc_gpu_matrix *m, const float *x_dev, int ck, int
 
 /* top_k_heap (cpu_mkl_baseline.cpp:128-149) over a device score vector. */
 int acc_gpu_top_k_heap(const float *y_dev, int n, int k, acc_element *out_dev, void *stream);
+
 
 /* Host-pointer convenience forms: copy in, run on the handle's device,
  * copy out, synchronise. */

@@ -67,8 +67,76 @@ def test_malformed_files_refused(acc, tmp_path):
     corrupt(badk, off, np.int32(200).tobytes())
     with pytest.raises(acc.AccelesError, match="not a packet file"):
         acc.PackedMatrix.load(badk)
-    corrupt(badk, off, np.int32(-1).tobytes())  # a padding lane is fine
-    acc.PackedMatrix.load(badk)
+    # a padding lane in the middle of a partition drops a row: refused
+    corrupt(badk, off, np.int32(-1).tobytes())
+    with pytest.raises(acc.AccelesError, match="not a packet file"):
+        acc.PackedMatrix.load(badk)
+
+
+def packet_offsets(pk, split):
+    """Byte offsets of the vectors of a packet file (each: a uint64 length, then the data)."""
+    ns = pk.info.nslices
+    o = {"part_row_begin": 72}
+    o["part_slice_begin"] = o["part_row_begin"] + 8 + 4 * (split + 1)
+    o["slice_off"] = o["part_slice_begin"] + 8 + 8 * (split + 1)
+    o["slice_ndw"] = o["slice_off"] + 8 + 4 * ns
+    o["perm"] = o["slice_ndw"] + 8 + 4 * ns
+    o["words"] = o["perm"] + 8 + 4 * ns * 64
+    return o
+
+
+def test_malformed_packet_fields_refused(acc, tmp_path):
+    """Every field the kernels or the host index with is checked on load."""
+    csr = acc.CSRMatrix.generate(1000, 50, deg=(1, 10), seed=3)
+    split = 4
+    pk = csr.pack(bits=8, split=split)
+    ns = pk.info.nslices
+    o = packet_offsets(pk, split)
+    good = str(tmp_path / "good.pk")
+    pk.save(good)
+    acc.PackedMatrix.load(good)
+    cases = {
+        # part_slice_begin: slice ranges must grow from 0 and give each partition ceil(rows_p/64) slices
+        "slices_jump": (o["part_slice_begin"] + 8 + 8, np.int64(ns + 1000)),
+        "slices_start": (o["part_slice_begin"] + 8, np.int64(1)),
+        # part_row_begin: from 0, never decreasing, ending at rows
+        "rows_start": (o["part_row_begin"] + 8, np.int32(5)),
+        "rows_decrease": (o["part_row_begin"] + 8 + 4 * 2, np.int32(10)),
+        # a vector length past what the file holds (no allocation is attempted)
+        "huge_length": (o["words"], np.uint64(1 << 60)),
+        "huge_slices": (o["slice_off"], np.uint64((1 << 63) - 1)),
+        # a row named twice (lane 1 repeats lane 0's row)
+        "row_twice": (o["perm"] + 8 + 4, None),
+        # an entry whose column lies past cols (column field of lane 0's first entry = 63 >= 50)
+        "column_range": (o["words"] + 8, None),
+    }
+    for name, (off, val) in cases.items():
+        bad = str(tmp_path / f"{name}.pk")
+        with open(good, "rb") as f:
+            data = bytearray(f.read())
+        if name == "row_twice":
+            data[off:off + 4] = data[off - 4:off]
+        elif name == "column_range":
+            w = int(np.frombuffer(bytes(data[off:off + 4]), np.uint32)[0])
+            data[off:off + 4] = np.uint32((w & ~0x3F) | 63).tobytes()  # col_bits = 6 for 50 columns
+        else:
+            data[off:off + val.nbytes] = val.tobytes()
+        with open(bad, "wb") as f:
+            f.write(bytes(data))
+        with pytest.raises(acc.AccelesError, match="not a packet file"):
+            acc.PackedMatrix.load(bad)
+
+
+def test_loaded_packet_statistics_recomputed(acc, tmp_path):
+    """The kept-entry count and the repeated-column statistics come from the
+    decoded entries, not from the header."""
+    csr = acc.CSRMatrix.generate(500, 40, deg=(1, 10), seed=4)
+    pk = csr.pack(bits=8, split=2)
+    path = str(tmp_path / "m.pk")
+    pk.save(path)
+    corrupt(path, 8 + 4 * 8 + 8, np.int64(12345).tobytes())  # nnz_kept in the header
+    got = acc.PackedMatrix.load(path)
+    assert got.info.nnz_kept == pk.info.nnz_kept
 
 
 def test_missing_file(acc, tmp_path):

@@ -98,3 +98,46 @@ def test_server_poisson_load(acc, gpu):
     assert st["completed"] == st["submitted"] > 1000
     assert 0 < st["p50_us"] <= st["p99_us"] <= st["max_us"]
     srv.close()
+
+
+def test_concurrent_add_matrix_admission(acc, gpu):
+    """Two threads add a matrix each to a server whose budget holds one: the
+    reservation is taken under the lock, so exactly one is admitted and the
+    other is refused with ACC_ERR_BUDGET."""
+    import threading
+    # 2048 columns at 8 bits: scores may pass 2^25, so the dense layout is
+    # never used and a matrix's footprint does not depend on the room left
+    csrs = [acc.CSRMatrix.generate(20000, 2048, deg=(40, 100), seed=700 + i) for i in range(2)]
+    pks = [c.pack(bits=8, split=32) for c in csrs]
+    probe = acc.Server(max_batch=16, slots=1, ck=512)
+    base = probe.stats()["device_bytes"]
+    probe.add_matrix(csrs[0], pks[0])
+    one = probe.stats()["device_bytes"] - base
+    probe.close()
+    srv = acc.Server(memory_budget=base + int(1.5 * one), max_batch=16, slots=1, ck=512)
+    results = [None, None]
+
+    def add(i):
+        try:
+            results[i] = srv.add_matrix(csrs[i], pks[i])
+        except acc.AccelesError as e:
+            results[i] = e
+
+    th = [threading.Thread(target=add, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    ok = [r for r in results if isinstance(r, int)]
+    bad = [r for r in results if isinstance(r, acc.AccelesError)]
+    # never both: the second reservation or its resize meets the first's bytes
+    assert len(ok) <= 1 and len(ok) + len(bad) == 2, results
+    assert all(e.code == -8 for e in bad)  # ACC_ERR_BUDGET
+    assert srv.stats()["device_bytes"] <= base + int(1.5 * one)
+    if not ok:  # both refused while each held its estimate: room for one remains
+        ok = [srv.add_matrix(csrs[0], pks[0])]
+    with pytest.raises(acc.AccelesError, match="budget"):
+        srv.add_matrix(csrs[1], pks[1])
+    x = acc.generate_queries(1, 2048, seed=1)[0]
+    assert len(srv.wait(srv.submit(ok[0], x, 10), 10)) == 10
+    srv.close()
