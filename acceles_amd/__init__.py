@@ -173,6 +173,7 @@ SIGNATURES = [
     ("acc_gpu_scores_lowbit", C.c_int, [_P, _P, _P, _P]),
     ("acc_gpu_candidates_lowbit", C.c_int, [_P, _P, C.c_int, _P, _P, C.POINTER(C.c_int), _P]),
     ("acc_gpu_top_k_heap", C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
+    ("acc_gpu_top_k_heap_noalloc", C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
     ("acc_topk_fp32_host", C.c_int, [_P, _F32P, C.c_int, _EP]),
     ("acc_topk_lowbit_host", C.c_int, [_P, _F32P, C.c_int, C.c_int, _EP]),
     ("acc_top_k_spmv", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _EP, C.c_int]),
@@ -794,3 +795,7 @@ def top_k_heap(y, n, k, out, stream=None):
     """top_k_heap (cpu_mkl_baseline.cpp:128-149) on a device score vector."""
     _check(lib().acc_gpu_top_k_heap(_ptr(y), n, k, _ptr(out), _stream(stream)))
 
+
+def top_k_heap_noalloc(y, n, k, out, stream=None):
+    """top_k_heap with its selection scratch in a static device array: allocates nothing."""
+    _check(lib().acc_gpu_top_k_heap_noalloc(_ptr(y), n, k, _ptr(out), _stream(stream)))

@@ -1086,6 +1086,58 @@ extern "C" int acc_gpu_top_k_heap(const float *y_dev, int n, int k, acc_element 
 }
 
 // ---------------------------------------------------------------------------
+// top_k_heap without allocation: selection scratch in a static device array
+// ---------------------------------------------------------------------------
+
+// Keys of the static selection scratch: [0] the shared threshold, [1] the
+// candidate count (low 32 bits), then the candidate keys (at most 512
+// workgroups x 960 keys, or the radix passes' keys when they fit).  3.75 MiB
+// per device, held until the process ends.
+static constexpr int kNoAllocBlocks = 512;
+static constexpr size_t kScratchKeys = 2 + (size_t)kNoAllocBlocks * kMaxK;
+static __device__ __align__(256) uint64_t scratch[kScratchKeys];
+
+// Device address of `scratch` on the current device (looked up once per
+// device; the array's host-side name is not a device address).
+static int scratch_address(uint64_t **out) {
+    static std::mutex mu;
+    static std::vector<uint64_t *> addr;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)dev >= addr.size()) addr.resize((size_t)dev + 1, nullptr);
+    if (!addr[(size_t)dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(ACC_ERR_DEVICE, "hipGetSymbolAddress returned a null address");
+        addr[(size_t)dev] = (uint64_t *)p;
+    }
+    *out = addr[(size_t)dev];
+    return ACC_OK;
+}
+
+extern "C" int acc_gpu_top_k_heap_noalloc(const float *y_dev, int n, int k, acc_element *out_dev, void *stream) {
+    if (!out_dev || (n > 0 && !y_dev) || n < 0 || k < 1 || k > kMaxK)
+        return fail(ACC_ERR_ARG, "bad arguments (n >= 0, k in [1, 960])");
+    uint64_t *s = nullptr;
+    int rc = scratch_address(&s);
+    if (rc != ACC_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    // at most kNoAllocBlocks workgroups, each appending at most k keys
+    const int chunk = std::max(8192, (int)(((int64_t)n + kNoAllocBlocks - 1) / kNoAllocBlocks));
+    const int blocks = std::max(1, (int)(((int64_t)n + chunk - 1) / chunk));
+    if ((size_t)blocks * k > kScratchKeys - 2) return fail(ACC_ERR_RANGE, "selection scratch too small for n and k");
+    uint64_t *thr = s, *cand = s + 2;
+    uint32_t *cnt = (uint32_t *)(s + 1);
+    // the previous call may have left any state: the threshold and count
+    // start at zero (a kernel, not a memset, so the call can be captured)
+    hipLaunchKernelGGL(get_select_reset(), dim3(1), dim3(64), 0, st, thr, cnt);
+    rc = check_launch();
+    if (rc != ACC_OK) return rc;
+    return launch_select(y_dev, n, k, blocks, chunk, thr, cnt, cand, out_dev, st, nullptr, cand, kScratchKeys - 2);
+}
+
+// ---------------------------------------------------------------------------
 // autotuner for ACC_BATCH_AUTO
 // ---------------------------------------------------------------------------
 

@@ -428,6 +428,16 @@ int acc_gpu_candidates_lowbit(acc_gpu_matrix *m, const float *x_dev, int ck, int
 /* top_k_heap (cpu_mkl_baseline.cpp:128-149) over a device score vector. */
 int acc_gpu_top_k_heap(const float *y_dev, int n, int k, acc_element *out_dev, void *stream);
 
+/* acc_gpu_top_k_heap without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * selection scratch is a static device array of 3.75 MiB per GPU, held until
+ * the process ends; limits as acc_gpu_top_k_heap (0 <= n < 2^31,
+ * 1 <= k <= 960).  Every call on one GPU shares that GPU's array and resets
+ * the state it needs at the start (by a kernel, so the call can be captured):
+ * calls that overlap in time on one GPU race and must be serialised by the
+ * caller (one stream). */
+int acc_gpu_top_k_heap_noalloc(const float *y_dev, int n, int k, acc_element *out_dev, void *stream);
 
 /* Host-pointer convenience forms: copy in, run on the handle's device,
  * copy out, synchronise. */
