@@ -11,6 +11,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -435,4 +436,99 @@ double orc_ndcg_score(const int32_t *sw_idx, const double *sw_val, int nsw, cons
     }
     for (int i = 0; i < nsw; ++i) idcg += sw_val[i] / log2((double)i + 2.0);
     return dcg / idcg;
+}
+
+/* ------------------------------------------------------------------ */
+/* all-core CPU baselines (timing legs beside the one-thread ports)    */
+/* ------------------------------------------------------------------ */
+
+/* top_k_spmv (cpu_mkl_baseline.cpp:249-275) on nthreads cores: the SpMV
+ * split into row ranges (one fmaf chain per row, as the one-thread port),
+ * then the heap Top-k of each thread's rows, then the Top-k of the threads'
+ * lists.  The same rows and scores as orc_top_k_spmv_f32 (the ranking order
+ * is a total order).  The reference's own "32 core" mode
+ * (cpu_mkl_baseline.cpp:195-245) times only thread creation and is not
+ * reproduced. */
+void orc_top_k_spmv_f32_par(int32_t nrows, const int64_t *ptr, const int32_t *idx, const float *val,
+                            const float *x, int k, int nthreads, float *y, int32_t *out_idx, float *out_val) {
+    if (nthreads < 1) nthreads = 1;
+    orc_item *lists = (orc_item *)malloc(sizeof(orc_item) * (size_t)nthreads * (size_t)(k > 0 ? k : 1));
+    int *counts = (int *)calloc((size_t)nthreads, sizeof(int));
+#pragma omp parallel num_threads(nthreads)
+    {
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+        const int32_t b = (int32_t)((int64_t)nrows * t / nt), e = (int32_t)((int64_t)nrows * (t + 1) / nt);
+        for (int32_t r = b; r < e; ++r) y[r] = dot_grouped(ptr[r], ptr[r + 1], idx, val, x, 1);
+        orc_item *h = lists + (size_t)t * (size_t)(k > 0 ? k : 1);
+        int hn = 0;
+        for (int32_t i = b; i < e; ++i) {
+            orc_item it = {y[i], i};
+            if (hn < k) { h[hn] = it; sift_up(h, hn); ++hn; }
+            else if (better(it, h[0])) { h[0] = it; sift_down(h, hn, 0); }
+        }
+        counts[t] = hn;
+    }
+    int total = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        memmove(lists + total, lists + (size_t)t * (size_t)(k > 0 ? k : 1), sizeof(orc_item) * (size_t)counts[t]);
+        total += counts[t];
+    }
+    orc_item *out = (orc_item *)malloc(sizeof(orc_item) * (size_t)(k > 0 ? k : 1));
+    const int m = heap_topk(lists, total, k, out);
+    for (int i = 0; i < k; ++i) {
+        out_idx[i] = i < m ? out[i].r : -1;
+        out_val[i] = i < m ? (float)out[i].s : -INFINITY;
+    }
+    free(out);
+    free(counts);
+    free(lists);
+}
+
+/* orc_two_stage on nthreads cores: the INT32 SpMV over row ranges and the
+ * per-partition selection over partitions in parallel, then the rescoring
+ * of the candidates (the same results as orc_two_stage). */
+int orc_two_stage_par(int32_t nrows, int32_t ncols, const int64_t *ptr, const int32_t *idx, const float *val,
+                      const int32_t *qv, const float *x, int G, int split, int ck, int k, int nthreads,
+                      int32_t *cand_rows, int32_t *out_idx, float *out_val) {
+    if (split < 1 || ck < split || nrows < split) return -1;
+    if (nthreads < 1) nthreads = 1;
+    int32_t *qx = (int32_t *)malloc(sizeof(int32_t) * (size_t)ncols);
+    float sx;
+    orc_quantize(ncols, x, 8, 1, &sx, qx);
+    int32_t *y = (int32_t *)malloc(sizeof(int32_t) * (size_t)nrows);
+    int bad = 0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(| : bad)
+    for (int32_t r = 0; r < nrows; ++r) {
+        int64_t acc = 0;
+        for (int64_t j = ptr[r]; j < ptr[r + 1]; ++j) acc += (int64_t)qv[j] * (int64_t)qx[idx[j]];
+        if (acc > INT32_MAX || acc < INT32_MIN) bad = 1;
+        y[r] = (int32_t)acc;
+    }
+    int n = -1;
+    if (!bad) {
+        const int32_t size = nrows / split;
+        const int m = ck / split;
+        int *got = (int *)malloc(sizeof(int) * (size_t)split);
+        orc_item *outs = (orc_item *)malloc(sizeof(orc_item) * (size_t)split * (size_t)m);
+#pragma omp parallel num_threads(nthreads)
+        {
+            orc_item *items = (orc_item *)malloc(sizeof(orc_item) * (size_t)(nrows - size * (split - 1)));
+#pragma omp for schedule(dynamic)
+            for (int p = 0; p < split; ++p) {
+                const int32_t b = p * size, e = (p == split - 1) ? nrows : b + size;
+                for (int32_t r = b; r < e; ++r) { items[r - b].s = y[r]; items[r - b].r = r; }
+                got[p] = heap_topk(items, e - b, m, outs + (size_t)p * (size_t)m);
+            }
+            free(items);
+        }
+        n = 0;
+        for (int p = 0; p < split; ++p)
+            for (int i = 0; i < got[p]; ++i) cand_rows[n++] = outs[(size_t)p * (size_t)m + (size_t)i].r;
+        orc_rescore_topk(ptr, idx, val, x, G, n, cand_rows, k, out_idx, out_val);
+        free(outs);
+        free(got);
+    }
+    free(y);
+    free(qx);
+    return n;
 }

@@ -157,6 +157,15 @@ void orc_top_k_spmv_f32(int32_t nrows, const int64_t *ptr, const int32_t *idx,
                         const float *val, const float *x, int k, float *y,
                         int32_t *out_idx, float *out_val);
 
+/* All-core timing legs (OpenMP), the same results as orc_top_k_spmv_f32 and
+ * orc_two_stage: the CPU baselines of the benchmark beside the one-thread
+ * ports.  Pinned by equality with the one-thread functions. */
+void orc_top_k_spmv_f32_par(int32_t nrows, const int64_t *ptr, const int32_t *idx, const float *val,
+                            const float *x, int k, int nthreads, float *y, int32_t *out_idx, float *out_val);
+int orc_two_stage_par(int32_t nrows, int32_t ncols, const int64_t *ptr, const int32_t *idx, const float *val,
+                      const int32_t *qv, const float *x, int G, int split, int ck, int k, int nthreads,
+                      int32_t *cand_rows, int32_t *out_idx, float *out_val);
+
 #ifdef __cplusplus
 }
 #endif

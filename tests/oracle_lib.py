@@ -66,6 +66,10 @@ def oracle() -> C.CDLL:
         L.orc_resparse_mask.restype = None
         L.orc_ndcg_score.argtypes = [_I32, _F64, C.c_int, _I32, C.c_int]
         L.orc_ndcg_score.restype = C.c_double
+        L.orc_top_k_spmv_f32_par.argtypes = [C.c_int32, _I64, _I32, _F32, _F32, C.c_int, C.c_int, _F32, _I32, _F32]
+        L.orc_top_k_spmv_f32_par.restype = None
+        L.orc_two_stage_par.argtypes = [C.c_int32, C.c_int32, _I64, _I32, _F32, _I32, _F32, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, _I32, _I32, _F32]
         _orc = L
     return _orc
 
@@ -232,6 +236,30 @@ def top_k_spmv_f32(ptr, idx, val, x, k):
     ov = np.empty(k, np.float32)
     oracle().orc_top_k_spmv_f32(n, ptr, idx, val, x, k, y, oi, ov)
     return oi, ov
+
+
+def top_k_spmv_f32_par(ptr, idx, val, x, k, nthreads):
+    """top_k_spmv on nthreads cores (the same results as top_k_spmv_f32)."""
+    n = len(ptr) - 1
+    y = np.empty(n, np.float32)
+    oi = np.empty(k, np.int32)
+    ov = np.empty(k, np.float32)
+    oracle().orc_top_k_spmv_f32_par(n, ptr, idx, val, x, k, nthreads, y, oi, ov)
+    return oi, ov
+
+
+def two_stage_par(ptr, idx, val, qv, x, cols, G, split, ck, k, nthreads):
+    """two_stage on nthreads cores (the same results)."""
+    n = len(ptr) - 1
+    cand = np.empty(ck, np.int32)
+    oi = np.empty(k, np.int32)
+    ov = np.empty(k, np.float32)
+    m = oracle().orc_two_stage_par(n, cols, np.ascontiguousarray(ptr, np.int64), np.ascontiguousarray(idx, np.int32),
+                                   np.ascontiguousarray(val, np.float32), np.ascontiguousarray(qv, np.int32),
+                                   np.ascontiguousarray(x, np.float32), G, split, ck, k, nthreads, cand, oi, ov)
+    if m < 0:
+        raise ValueError("two-stage failed")
+    return cand[:m], oi, ov
 
 
 def resparse_mask(ptr, val, cols, keep, mean_over_cells=False):

@@ -367,3 +367,29 @@ def test_ultracsr_roundtrip():
     np.testing.assert_array_equal(c, col)
     np.testing.assert_array_equal(v, qv)
     np.testing.assert_array_equal(xr - xr[0], row - row[0])  # row counter advances once per row
+
+
+@pytest.mark.parametrize("nthreads", [1, 3, 8])
+def test_all_core_legs_equal_one_thread(acc, nthreads):
+    """The all-core CPU baselines give the one-thread ports' results: the
+    FP32 Top-k (heap per thread, then a heap of the threads' lists; ties at
+    the cut resolved to the lower row in both) and the two-stage path."""
+    csr = acc.CSRMatrix.generate(30000, 200, deg=(5, 30), seed=nthreads)
+    ptr, idx, val = csr.arrays()
+    X = acc.generate_queries(3, 200, seed=nthreads)
+    for x in X:
+        for k in (1, 100, 700):
+            np.testing.assert_array_equal(np.stack(O.top_k_spmv_f32_par(ptr, idx, val, x, k, nthreads)),
+                                          np.stack(O.top_k_spmv_f32(ptr, idx, val, x, k)))
+    # ties everywhere: every score equal, the lowest rows win
+    ones = np.ones_like(val)
+    xo = np.ones(200, np.float32)
+    oi, _ = O.top_k_spmv_f32_par(np.arange(0, 30001, dtype=np.int64), np.zeros(30000, np.int32),
+                                 ones[:30000], xo, 50, nthreads)
+    np.testing.assert_array_equal(oi, np.arange(50))
+    q, _ = O.quantize(val, 8, False)
+    for x in X:
+        a = O.two_stage_par(ptr, idx, val, q, x, 200, 4, 16, 256, 100, nthreads)
+        b = O.two_stage(ptr, idx, val, q, x, 200, 4, 16, 256, 100)
+        for u, v in zip(a, b):
+            np.testing.assert_array_equal(u, v)
