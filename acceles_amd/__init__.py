@@ -709,7 +709,7 @@ class GpuMatrix:
 
     @property
     def info(self):
-        """acc_gpu_info_t, read afresh (batch mode and dense tiles change after upload)."""
+        """acc_gpu_info_t, read afresh (the batch mode changes after upload)."""
         info = GpuInfo()
         _check(lib().acc_gpu_matrix_info(self._h, C.byref(info)))
         return info

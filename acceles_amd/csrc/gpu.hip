@@ -27,8 +27,8 @@ static constexpr int kMaxM = 64;     // rows kept per partition
 static constexpr int kMaxK = 960;    // Top-K limit of the FP32 path (collector cap 1024)
 static constexpr int kMaxCK = 4096;  // candidates of the low-bit path
 static constexpr size_t kWordPad = 32 * 64;  // dwords past the packets (>= widest group, 22 x 64)
-static constexpr size_t kDenseLds = 160 * 1024 - 1024;  // LDS of a dense_scores workgroup
-static constexpr int64_t kMallBytes = 240ll << 20;      // dense tiles read with normal loads (256 MB MALL)
+static constexpr int kPkMaxCols = 1024;  // matrix-core scores: B operands of 32 column blocks in registers
+static constexpr int kPkMaxE = 18;       // entry widths instantiated for them
 
 // Per-batch device workspace of the batched path.  The handle owns one;
 // every plan (captured graph) owns its own, so replaying a plan never touches
@@ -97,21 +97,19 @@ struct acc_gpu_matrix {
     int32_t *d_crow = nullptr, *d_cscore = nullptr;  // split * kMaxM (candidate outputs)
     int64_t *d_part_slice = nullptr;                 // split + 1
     BatchWs ws;                      // batched-path workspace (grown on demand)
-    // dense int8 tiles of the batched path (matrix cores), built on demand
+    // batched path on the matrix cores (slices expanded from the packets in LDS)
     uint8_t *d_nvalid = nullptr;     // valid rows per slice
-    bool dense_ok = false;           // the layout can hold this matrix exactly
+    bool dense_ok = false;           // the matrix cores can score this matrix exactly
     std::string dense_why;           // why not, otherwise
-    bool has_dense = false;
     int batch_mode = ACC_BATCH_AUTO;
-    int dense_min_nq = 16;           // ACC_BATCH_AUTO, untuned: dense tiles from this batch size on
+    int dense_min_nq = 16;           // ACC_BATCH_AUTO, untuned: matrix cores from this batch size on
+    bool dup = false;                // some row names a column twice
     // ACC_BATCH_AUTO after acc_gpu_autotune: the layout measured fastest at
     // tune_nq[i], used for batches of tune_nq[i] .. tune_nq[i+1]-1 queries
     int ntune = 0;
     int tune_nq[16] = {0};
     int tune_layout[16] = {0};
     int cols_pad = 0, nkb = 0, u8off = 0;
-    int8_t *d_tiles = nullptr;
-    int64_t dense_bytes = 0;
     // profiling events around the launches of the last query
     bool profiling = false;
     hipEvent_t ev[12] = {};  // start and end of each kernel of the last query
@@ -387,7 +385,6 @@ DenseArgs dense_args(const acc_gpu_matrix *m) {
     a.perm = m->d_perm;
     a.part_slice = m->d_part_slice;
     a.nvalid = m->d_nvalid;
-    a.tiles = m->d_tiles;
     a.nslices = m->nslices;
     a.cols = m->cols;
     a.cols_pad = m->cols_pad;
@@ -396,32 +393,13 @@ DenseArgs dense_args(const acc_gpu_matrix *m) {
     a.val_bits = m->bits;
     a.is_signed = m->is_signed;
     a.u8off = m->u8off;
+    a.dup = m->dup ? 1 : 0;
     a.split = m->split;
     a.ptr = m->d_ptr;
     a.idx = m->d_idx;
     a.val = m->d_val;
     a.G = m->group;
     return a;
-}
-
-// Dense int8 tiles from the sliced packets on the device (dense_build).
-int build_dense(acc_gpu_matrix *m) {
-    if (m->has_dense) return ACC_OK;
-    if (!m->dense_ok) return fail(ACC_ERR_ARG, "dense tiles unavailable: " + m->dense_why);
-    const size_t bytes = (size_t)m->nslices * 2 * m->nkb * 1024;
-    int rc = dalloc(m, &m->d_tiles, bytes);
-    if (rc != ACC_OK) return rc;
-    DenseArgs a = dense_args(m);
-    const size_t lds = (size_t)m->nkb * 1024;
-    rc = lds_limit((const void *)get_dense_build(), lds);
-    if (rc != ACC_OK) return rc;
-    hipLaunchKernelGGL(get_dense_build(), dim3((unsigned)m->nslices), dim3(64), lds, 0, a);
-    rc = check_launch();
-    if (rc != ACC_OK) return rc;
-    HIP_TRY(hipDeviceSynchronize());
-    m->dense_bytes = (int64_t)bytes;
-    m->has_dense = true;
-    return ACC_OK;
 }
 
 // Layout of a batch of nq queries: ACC_BATCH_SINGLE, _SPARSE or _DENSE.
@@ -436,18 +414,36 @@ int batch_layout(const acc_gpu_matrix *m, int nq) {
     return m->dense_ok && nq >= m->dense_min_nq ? ACC_BATCH_DENSE : ACC_BATCH_SPARSE;
 }
 
-// Batched low-bit queries on the dense tiles: dense_prep, dense_scores,
-// dense_select (which also rescores when out != null), then batch_topk.
+// Value mode of the image bytes (k_pkmfma.hpp): 1 unsigned 8-bit (stored as
+// v - 128), 2 signed below 8 bits (sign-extended), 0 the field as is.
+int pk_value_mode(const acc_gpu_matrix *m) {
+    if (m->u8off) return 1;
+    return m->is_signed && m->bits < 8 ? 2 : 0;
+}
+
+// B registers for 8, 16 or 32 column blocks: the smallest that holds the
+// matrix (the multiplications run over all of them, zero past nkb).
+dense_fn pk_kernel(const acc_gpu_matrix *m) {
+    const int E = m->entry_bits, vm = pk_value_mode(m);
+    if (m->nkb > 16) return get_pk_scores_32(E, vm);
+    return m->nkb > 8 ? get_pk_scores_16(E, vm) : get_pk_scores_8(E, vm);
+}
+
+// Batched low-bit queries on the matrix cores: dense_prep, pk_scores (slices
+// expanded from the packets in LDS), dense_select (which also rescores when
+// out != null), then batch_topk.
 int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, int kmax, acc_element *out,
                  int out_stride, int32_t *crow, int32_t *cscore, hipStream_t st) {
-    int rc = build_dense(m);
-    if (rc != ACC_OK) return rc;
-    int NT = nq <= 32 ? 1 : (nq <= 64 ? 2 : 4);
-    while (NT > 1 && (size_t)m->nkb * NT * 1024 + 128 * NT > kDenseLds) NT >>= 1;
-    const int QB = 32 * NT;
-    const int nqb = (nq + QB - 1) / QB;
+    if (!m->dense_ok) return fail(ACC_ERR_ARG, "matrix-core scores unavailable: " + m->dense_why);
+    // query blocks of at most kPkMfmaWaves tiles of 32 queries (one MFMA
+    // wavefront each), balanced; every block has kPkDecodeWaves decode
+    // wavefronts
+    const int tiles = (nq + 31) / 32;
+    const int nqb = (tiles + kPkMfmaWaves - 1) / kPkMfmaWaves;
+    const int NT = (tiles + nqb - 1) / nqb, QB = 32 * NT;
+    const int W = NT + kPkDecodeWaves;
     const size_t cand = (size_t)nq * m->split * M;
-    rc = ws_grow(ws, &ws.qtab, &ws.qtab_n, (size_t)nqb * m->nkb * NT * 1024);
+    int rc = ws_grow(ws, &ws.qtab, &ws.qtab_n, (size_t)nqb * m->nkb * NT * 1024);
     if (rc == ACC_OK) rc = ws_grow(ws, &ws.qplain, &ws.qplain_n, (size_t)nq * m->cols_pad);
     if (rc == ACC_OK) rc = ws_grow(ws, &ws.corr, &ws.corr_n, (size_t)nqb * QB);
     if (rc == ACC_OK) rc = ws_grow(ws, &ws.tk, &ws.tk_n, (size_t)nq * m->nslices * 4);
@@ -472,16 +468,18 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
 
     LAUNCH(m, 0, get_dense_prep(), dim3(nqb * QB), dim3(ACC_WG), 0, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
-    // non-temporal tile loads only for one query block past the MALL size:
-    // with several blocks, the sibling blocks of a slice range run on one XCD
-    // and find the tiles in its L2
-    dense_fn fs = get_dense_scores(NT, m->dense_bytes > kMallBytes && nqb == 1);
-    const size_t lds_s = (size_t)m->nkb * NT * 1024 + (size_t)QB * 4;
+    dense_fn fs = pk_kernel(m);
+    if (!fs) return fail(ACC_ERR_ARG, "unsupported entry width");
+    const size_t lds_s = (size_t)2 * 64 * (m->cols_pad + 16) + kPkLdsTail;  // two slice images and the tail
     if ((rc = lds_limit((const void *)fs, lds_s)) != ACC_OK) return rc;
-    const int64_t groups = (m->nslices + 7) / 8;
-    // a multiple of 8: the kernel keeps a slice range's query blocks on one XCD
-    const int nwx = (int)std::max<int64_t>(8, std::min<int64_t>(groups, std::max(1, m->cus / nqb)) / 8 * 8);
-    LAUNCH(m, 1, fs, dim3(nwx * nqb), dim3(512), lds_s, st, a);
+    // workgroups per query block: as many as fit the chip at once, a multiple
+    // of 8 (the kernel keeps a slice range's query blocks on one XCD)
+    int occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fs, W * 64, lds_s) != hipSuccess || occ < 1)
+        occ = 1;
+    const int64_t fit = (int64_t)m->cus * occ / nqb;
+    const int nwx = (int)std::max<int64_t>(8, std::min<int64_t>(fit, (m->nslices + 7) / 8 * 8) / 8 * 8);
+    LAUNCH(m, 1, fs, dim3(nwx * nqb), dim3(W * 64), lds_s, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
     // per wavefront: two key buffers, the int8 query (rescans), the FP32 query (rescoring)
     const size_t lds_sel = (size_t)ACC_WAVES * 256 * 8 + (size_t)ACC_WAVES * m->cols_pad * 5;
@@ -790,17 +788,21 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         m->cols_pad = (m->cols + 31) & ~31;
         m->nkb = m->cols_pad / 32;
         m->u8off = (!m->is_signed && m->bits == 8) ? 1 : 0;
+        m->dup = pk->dup_rows > 0;
         {
-            // a tile byte holds the sum of a row's values in one column (rows
-            // may name a column twice): v - 128 for unsigned 8-bit, else v
+            // a byte of the slice image holds the sum of a row's values in one
+            // column (rows may name a column twice): v - 128 for unsigned
+            // 8-bit, else v
             const int64_t lo = m->u8off ? 0 : -128, hi = m->u8off ? 255 : 127;
             const int64_t maxv = std::max<int64_t>({(int64_t)pk->merged_max, -(int64_t)pk->merged_min, 1});
             if (pk->merged_min < lo || pk->merged_max > hi)
                 m->dense_why = "a row's values in one column sum past one byte (a row names a column twice)";
             else if (maxv * 128 * (int64_t)m->cols >= (int64_t(1) << 25))
                 m->dense_why = "scores may reach 2^25 (too many columns for the value width)";
-            else if ((size_t)m->nkb * 1024 + 128 > kDenseLds)
-                m->dense_why = "more than 4960 columns";
+            else if (m->cols > kPkMaxCols)
+                m->dense_why = "more than 1024 columns";
+            else if (m->entry_bits > kPkMaxE)
+                m->dense_why = "entries wider than 18 bits";
             else
                 m->dense_ok = true;
         }
@@ -836,7 +838,7 @@ extern "C" int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info
     info->device_bytes = m->device_bytes;
     info->batch_mode = m->batch_mode;
     info->dense_ok = m->dense_ok ? 1 : 0;
-    info->dense_bytes = m->dense_bytes;
+    info->dense_bytes = 0;
     return ACC_OK;
 }
 
@@ -851,12 +853,7 @@ extern "C" int acc_gpu_set_batch_mode(acc_gpu_matrix *m, int mode, int min_dense
         return fail(ACC_ERR_ARG, "mode must be ACC_BATCH_AUTO, _SINGLE, _SPARSE or _DENSE");
     if (min_dense_batch < 0) return fail(ACC_ERR_ARG, "min_dense_batch must be >= 0");
     if (mode != ACC_BATCH_SPARSE && !m->has_packed) return fail(ACC_ERR_ARG, "matrix was uploaded without low-bit packets");
-    if (mode == ACC_BATCH_DENSE && !m->dense_ok) return fail(ACC_ERR_ARG, "dense tiles unavailable: " + m->dense_why);
-    HIP_TRY(hipSetDevice(m->device));
-    if ((mode == ACC_BATCH_DENSE || mode == ACC_BATCH_AUTO) && m->dense_ok) {
-        int rc = build_dense(m);
-        if (rc != ACC_OK) return rc;
-    }
+    if (mode == ACC_BATCH_DENSE && !m->dense_ok) return fail(ACC_ERR_ARG, "matrix-core scores unavailable: " + m->dense_why);
     m->batch_mode = mode;
     if (min_dense_batch > 0) {
         m->dense_min_nq = min_dense_batch;
@@ -1150,7 +1147,6 @@ extern "C" int acc_gpu_autotune(acc_gpu_matrix *m, int ck, int k, int max_batch,
     HIP_TRY(hipSetDevice(m->device));
     // the dense layout competes unless the caller pinned the packet or single-query layout
     const bool try_dense = m->dense_ok && m->batch_mode != ACC_BATCH_SPARSE && m->batch_mode != ACC_BATCH_SINGLE;
-    if (try_dense && (rc = build_dense(m)) != ACC_OK) return rc;
     std::vector<float> xh((size_t)max_batch * m->cols);
     acc_generate_queries(max_batch, m->cols, ACC_VAL_NORMAL, 0, 0x7E57ull, xh.data());
     float *x = nullptr;

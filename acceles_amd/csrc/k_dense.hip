@@ -1,19 +1,19 @@
-// k_dense.hip -- batched low-bit path on dense int8 tiles and matrix cores.
+// k_dense.hip -- the batched low-bit path around the matrix-core scores
+// (pk_scores, k_pkmfma.hpp): query preparation, per-partition selection with
+// FP32 rescoring, and the per-query Top-K.
 //
 // The INT32 stage-1 scores (quantify_util.py:173-196: exact integer dot of
-// the quantized rows with the quantized query) are computed for a block of
-// queries per matrix pass with v_mfma_i32_32x32x32_i8: at the densities of
-// the embedding workloads (2-15% non-zeros) a dense int8 tile on the matrix
-// cores costs less than decoding (column, value) packets on the vector ALUs
-// once a pass serves tens of queries.  Integer arithmetic is exact, so the
-// scores are the same integers as the packet kernels'.  Only the four best
-// (score, row) keys of each (query, 64-row slice) leave the kernel; the
-// per-partition selection derives its threshold from the slice maxima and
-// rescans exactly the rare slice whose list may have dropped a qualifying row.
+// the quantized rows with the quantized query) of a block of queries come
+// from v_mfma_i32_32x32x32_i8 over the slices' packets, expanded in LDS: at
+// the densities of the embedding workloads (2-15% non-zeros) the matrix cores
+// multiply a dense slice image faster than the vector ALUs decode (column,
+// value) pairs once a pass serves tens of queries.  Integer arithmetic is
+// exact, so the scores are the same integers as the packet kernels'.  Only
+// the four best (score, row) keys of each (query, 64-row slice) leave the
+// scores kernel; the per-partition selection derives its threshold from the
+// slice maxima and rescans exactly the rare slice whose list may have
+// dropped a qualifying row.
 #include "kernels_common.hpp"
-
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
 
 // 32-bit slice keys: (score + 2^25) << 6 | row_in_slice; a larger key is a
 // higher score (equal scores: any fixed order will do, the lists are only a
@@ -22,103 +22,40 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ int32_t tk_score(uint32_t k) { return (int32_t)(k >> 6) - (1 << 25); }
 __device__ __forceinline__ int tk_rho(uint32_t k) { return (int)(k & 63u); }
 
-__device__ __forceinline__ uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
-    uint32_t d;
-    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
-    return d;
-}
-
-// keeps k0 >= k1 >= k2 >= k3 the four largest of what was offered: with the
-// list ordered, the new i-th entry is the median of the old (i-1)-th, i-th
-// and x (one v_med3 each)
-__device__ __forceinline__ void top4_insert(uint32_t &k0, uint32_t &k1, uint32_t &k2, uint32_t &k3, uint32_t x) {
-    const uint32_t n3 = umed3(k2, k3, x), n2 = umed3(k1, k2, x), n1 = umed3(k0, k1, x);
-    k0 = max(k0, x);
-    k1 = n1;
-    k2 = n2;
-    k3 = n3;
-}
-
 // Exact INT32 score of this lane's row (row_in_slice = lane) of slice s for
-// one query, from the dense tiles with v_dot4_i32_i8: the row's 32 bytes of
-// column block kb are the 16-byte registers of lanes i and i+32 of tile
-// (kb, rt).  qx: the query's int8 values in LDS, plain order, cols_pad long;
-// corr: its unsigned-8-bit offset term.  Integer sums in any order are the
-// matrix cores' result.
-__device__ __forceinline__ int32_t slice_dot_dense(const DenseArgs &a, int64_t s, const int8_t *qx, int32_t corr) {
-    const int lane = lane_id(), rt = lane >> 5, i = lane & 31;
-    const size_t kstr = (size_t)a.nslices * 2048;  // bytes per column block
-    const int8_t *base = a.tiles + ((size_t)s * 2 + rt) * 1024 + i * 16;
-    const int *q4 = (const int *)qx;
-    int32_t acc = 0;
-    constexpr int U = 8;
-    for (int kb0 = 0; kb0 < a.nkb; kb0 += U) {
-        int4 lo[U], hi[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kb = min(kb0 + u, a.nkb - 1);
-            lo[u] = *(const int4 *)(base + kb * kstr);
-            hi[u] = *(const int4 *)(base + kb * kstr + 512);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int *qd = q4 + min(kb0 + u, a.nkb - 1) * 8;
-            if (kb0 + u >= a.nkb) continue;  // wave-uniform
-            acc = __builtin_amdgcn_sdot4(lo[u].x, qd[0], acc, false);
-            acc = __builtin_amdgcn_sdot4(lo[u].y, qd[1], acc, false);
-            acc = __builtin_amdgcn_sdot4(lo[u].z, qd[2], acc, false);
-            acc = __builtin_amdgcn_sdot4(lo[u].w, qd[3], acc, false);
-            acc = __builtin_amdgcn_sdot4(hi[u].x, qd[4], acc, false);
-            acc = __builtin_amdgcn_sdot4(hi[u].y, qd[5], acc, false);
-            acc = __builtin_amdgcn_sdot4(hi[u].z, qd[6], acc, false);
-            acc = __builtin_amdgcn_sdot4(hi[u].w, qd[7], acc, false);
-        }
-    }
-    return acc + corr;
-}
-
-// One wavefront per slice: decode each lane's packet stream into a 32-row
-// tile image in LDS (nkb KB), then copy it out.  Runs once per matrix.
-__global__ __launch_bounds__(64) void dense_build(DenseArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int lane = lane_id();
-    const int64_t s = blockIdx.x;
+// one query, decoded from the slice's packets with the entry width known
+// only at run time: the lane's dwords stream through a 64-bit bit buffer,
+// eight loads in flight, and every complete E-bit entry is taken off the
+// bottom (the same number of entries on every lane, so no divergence).
+// qx: the query's int8 values in LDS.
+__device__ __forceinline__ int32_t slice_dot_packets(const DenseArgs &a, int64_t s, const int8_t *qx) {
     const uint2 sl = a.slices[s];
-    const int E = a.col_bits + a.val_bits;
-    const uint32_t cmask = (1u << a.col_bits) - 1u, vmask = (1u << a.val_bits) - 1u;
-    const uint32_t fill = a.u8off ? 0x80808080u : 0u;
-    const int nkb = a.nkb;
-    uint4 *l4 = (uint4 *)smem;
-    int8_t *l1 = (int8_t *)smem;
-    for (int rt = 0; rt < 2; ++rt) {
-        for (int i = lane; i < nkb * 64; i += 64) l4[i] = make_uint4(fill, fill, fill, fill);
-        wave_sync();
-        if ((lane >> 5) == rt) {
-            const int i = lane & 31;
-            for (int j = 0;; ++j) {
-                const int pos = j * E, w = pos >> 5, sh = pos & 31;
-                if (w >= (int)sl.y) break;
-                const uint32_t lo = a.words[((size_t)sl.x + w) * 64 + lane];
-                const uint32_t hi = (w + 1 < (int)sl.y) ? a.words[((size_t)sl.x + w + 1) * 64 + lane] : 0u;
-                const uint32_t bits = (uint32_t)((((uint64_t)hi << 32) | lo) >> sh);
-                const uint32_t v = (bits >> a.col_bits) & vmask;
-                if (v == 0u) break;
-                const uint32_t c = bits & cmask;
-                int sv = (int)v;
-                if (a.is_signed && (v >> (a.val_bits - 1))) sv = (int)(v | ~vmask);
-                const int kb = (int)(c >> 5), h = (int)((c >> 4) & 1u), bb = (int)(c & 15u);
-                // added, not stored: a row that names a column twice keeps the
-                // sum (the host checked that every sum fits the byte; the
-                // cell starts at 0, or at -128 for the offset unsigned bytes)
-                int8_t &cell = l1[(kb * 64 + i + 32 * h) * 16 + bb];
-                cell = (int8_t)(cell + sv);
+    const int cb = a.col_bits, vb = a.val_bits, E = cb + vb;
+    const uint64_t emask = (1ull << E) - 1ull;
+    const uint32_t cmask = (1u << cb) - 1u;
+    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(a.words, sl);
+    const int lane = lane_id();
+    uint64_t buf = 0ull;
+    int nb = 0;
+    int32_t acc = 0;
+    for (int d0 = 0; d0 < (int)sl.y; d0 += 8) {
+        uint32_t w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, ((d0 + u) * 64 + lane) * 4, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            buf |= (uint64_t)w[u] << nb;
+            nb += 32;
+            while (nb >= E) {
+                const uint32_t e = (uint32_t)(buf & emask);
+                buf >>= E;
+                nb -= E;
+                const int v = a.is_signed ? __builtin_amdgcn_sbfe(e, cb, vb) : (int)__builtin_amdgcn_ubfe(e, cb, vb);
+                acc += v * (int)qx[e & cmask];
             }
         }
-        wave_sync();
-        uint4 *dst = (uint4 *)a.tiles + ((size_t)s * 2 + rt) * 64;
-        for (int i = lane; i < nkb * 64; i += 64) dst[(size_t)(i >> 6) * a.nslices * 128 + (i & 63)] = l4[i];
-        wave_sync();
     }
+    return acc;
 }
 
 // One workgroup per query slot of the padded batch: int8 quantization with
@@ -175,149 +112,6 @@ __global__ __launch_bounds__(ACC_WG) void dense_prep(DenseArgs a) {
         int tot = 0;
         for (int w = 0; w < ACC_WAVES; ++w) tot += isum[w];
         a.corr[q] = a.u8off ? 128 * tot : 0;
-    }
-}
-
-// Scores of 64-row slices against a block of QB = 32*NT queries.  A
-// workgroup serves one query block and the slices wx*8 + wave + k*8*nwx of
-// its slice range wx (grid = nwx * nqb, nwx a multiple of 8).
-// Per slice a wavefront runs 2*NT MFMA chains over the nkb column blocks, the
-// next P blocks of the matrix tiles in flight; the B operands come from the
-// query table in LDS.
-template <bool STREAM>
-__device__ __forceinline__ v4i tile_load(const v4i *p) {
-    if (STREAM) return __builtin_nontemporal_load(p);
-    return *p;
-}
-
-template <int NT, bool STREAM>
-__global__ __launch_bounds__(512) void dense_scores(DenseArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int QB = 32 * NT;
-    const int nkb = a.nkb, nqb = a.nqb;
-    // workgroups are dealt round-robin to the 8 XCDs: the nqb query blocks of
-    // one slice range (same wx) get ids b, b+8, b+16, ..., hence the same XCD,
-    // and the second and later blocks find the tiles in that XCD's L2
-    const int c = blockIdx.x % (8 * nqb);
-    const int qb = c / 8;
-    const int wx = (blockIdx.x / (8 * nqb)) * 8 + (c % 8), nwx = gridDim.x / nqb;
-    v4i *qt = (v4i *)smem;
-    int32_t *corr = (int32_t *)(smem + (size_t)nkb * NT * 1024);
-    {
-        // all loads of a round are issued before any LDS write, so the table
-        // arrives in one memory latency per 64 KB instead of one per 8 KB
-        const v4i *src = (const v4i *)(a.qtab + (size_t)qb * nkb * NT * 1024);
-        const int n = nkb * NT * 64;
-        constexpr int CU = 8;
-        for (int b0 = 0; b0 < n; b0 += 512 * CU) {
-            v4i tmp[CU];
-#pragma unroll
-            for (int u = 0; u < CU; ++u) {
-                const int i = b0 + u * 512 + (int)threadIdx.x;
-                tmp[u] = i < n ? src[i] : (v4i){0};
-            }
-#pragma unroll
-            for (int u = 0; u < CU; ++u) {
-                const int i = b0 + u * 512 + (int)threadIdx.x;
-                if (i < n) qt[i] = tmp[u];
-            }
-        }
-        for (int i = threadIdx.x; i < QB; i += 512) corr[i] = a.corr[qb * QB + i];
-    }
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = lane_id(), hi = lane >> 5;
-    const int64_t stride = (int64_t)nwx * 8;
-    int64_t s = (int64_t)wx * 8 + wave;
-    // Matrix blocks stream through a ring of RING register slots: the block of
-    // step k sits in slot k % RING and is loaded DIST steps ahead, into the
-    // slot whose MFMAs were issued RING - DIST steps before.  Loads are
-    // unconditional (indices clamped into the matrix; the few past the end are
-    // redundant re-reads) so each lands directly in its slot -- a conditional
-    // load would be copied into place after waiting for every outstanding load.
-    // The next slice's first DIST blocks are requested before the epilogue.
-    constexpr int RING = 4, DIST = 2;
-    v4i pa[RING], pb[RING];
-#pragma unroll
-    for (int j = 0; j < RING; ++j) {
-        pa[j] = (v4i){0};
-        pb[j] = (v4i){0};
-    }
-    // column block kb of all slices is contiguous: at a given step the
-    // wavefronts of the whole chip read neighbouring kilobytes
-    const size_t kstr = (size_t)a.nslices * 128;  // v4i per column block
-    const v4i *tiles = (const v4i *)a.tiles + lane;
-    if (s < a.nslices) {
-        const v4i *B0 = tiles + (size_t)s * 128;
-#pragma unroll
-        for (int j = 0; j < DIST; ++j) {
-            const size_t kj = (size_t)min(j, nkb - 1) * kstr;
-            pa[j] = tile_load<STREAM>(B0 + kj);
-            pb[j] = tile_load<STREAM>(B0 + kj + 64);
-        }
-    }
-    for (; s < a.nslices; s += stride) {
-        const v4i *A0 = tiles + (size_t)s * 128;
-        const v4i *A1 = A0 + 64;
-        v16i acc[2][NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            acc[0][t] = (v16i){0};
-            acc[1][t] = (v16i){0};
-        }
-        for (int k0 = 0; k0 < nkb; k0 += RING) {
-#pragma unroll
-            for (int j = 0; j < RING; ++j) {
-                const int k = k0 + j;
-                const size_t kn = (size_t)min(k + DIST, nkb - 1) * kstr;
-                pa[(j + DIST) % RING] = tile_load<STREAM>(A0 + kn);
-                pb[(j + DIST) % RING] = tile_load<STREAM>(A1 + kn);
-                if (k < nkb) {
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) {
-                        const v4i b = qt[(k * NT + t) * 64 + lane];
-                        acc[0][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(pa[j], b, acc[0][t], 0, 0, 0);
-                        acc[1][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(pb[j], b, acc[1][t], 0, 0, 0);
-                    }
-                }
-            }
-        }
-        {
-            const int64_t sn = min(s + stride, a.nslices - 1);  // past the end: a redundant in-bounds load
-            const v4i *B0 = tiles + (size_t)sn * 128;
-#pragma unroll
-            for (int j = 0; j < DIST; ++j) {
-                const size_t kj = (size_t)min(j, nkb - 1) * kstr;
-                pa[j] = tile_load<STREAM>(B0 + kj);
-                pb[j] = tile_load<STREAM>(B0 + kj + 64);
-            }
-        }
-        // D[i][j] of a 32x32 tile: lane j + 32*((i/4)%2), register (i%4) + 4*(i/8);
-        // row_in_slice = rt*32 + i = E(rt, r) + 4*hi.  Padding rows (the tail of a
-        // partition's last slice) hold no entries and score exactly 0 in every
-        // value mode; the selection discards their keys.
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int ql = t * 32 + (lane & 31);
-            // key = ((score + corr + 2^25) << 6) + row_in_slice, in 32-bit wrapping
-            // arithmetic (the true value is in [64, 2^32))
-            const uint32_t D = ((uint32_t)(corr[ql] + (1 << 25)) << 6) + (uint32_t)(hi * 4);
-            uint32_t k0 = 0u, k1 = 0u, k2 = 0u, k3 = 0u;
-#pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const uint32_t E = (uint32_t)(rt * 32 + (r >> 2) * 8 + (r & 3));
-                    top4_insert(k0, k1, k2, k3, ((uint32_t)acc[rt][t][r] << 6) + D + E);
-                }
-            const uint32_t o0 = (uint32_t)__shfl_xor((int)k0, 32), o1 = (uint32_t)__shfl_xor((int)k1, 32);
-            const uint32_t o2 = (uint32_t)__shfl_xor((int)k2, 32), o3 = (uint32_t)__shfl_xor((int)k3, 32);
-            top4_insert(k0, k1, k2, k3, o0);
-            top4_insert(k0, k1, k2, k3, o1);
-            top4_insert(k0, k1, k2, k3, o2);
-            top4_insert(k0, k1, k2, k3, o3);
-            const int q = qb * QB + ql;
-            if (hi == 0 && q < a.nq) *(uint4 *)(a.tk + ((size_t)q * a.nslices + s) * 4) = make_uint4(k0, k1, k2, k3);
-        }
     }
 }
 
@@ -378,8 +172,8 @@ __device__ __forceinline__ void list_rows(const DenseArgs &a, const uint4 (&k4)[
 }
 
 // Offers the listed keys of those slices scoring >= ts to the collector; a
-// slice whose list may have dropped such a row is rescanned exactly (from the
-// dense tiles).  Keys of padding rows are discarded, and since they may have
+// slice whose list may have dropped such a row is rescanned exactly (from its
+// packets).  Keys of padding rows are discarded, and since they may have
 // displaced real rows from a list, every list whose fourth key scores >= ts
 // is rescanned.
 template <int U>
@@ -413,7 +207,7 @@ __device__ __forceinline__ void offer_lists(const DenseArgs &a, WaveTopM<2> &cb,
                 have_qx = true;
             }
             const int64_t s = s0 + b + u * 64 + l;
-            const int32_t sc = slice_dot_dense(a, s, qx, a.corr[q]);
+            const int32_t sc = slice_dot_packets(a, s, qx);
             const int32_t r = a.perm[s * 64 + lane];
             cb.push(r >= 0 && sc >= ts, dkey_i32(sc, r));
         }
@@ -611,7 +405,7 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
                     wave_sync();
                     have_qx = true;
                 }
-                const int32_t sc = slice_dot_dense(a, s, qx, a.corr[q]);
+                const int32_t sc = slice_dot_packets(a, s, qx);
                 const int32_t r = a.perm[s * 64 + lane];
                 const bool pr = r >= 0 && sc >= ts;
                 const uint64_t bm = __ballot(pr);
@@ -719,19 +513,7 @@ __global__ __launch_bounds__(ACC_WG) void batch_topk_radix(DenseArgs a) {
     topk_radix<V>(a.resc + (size_t)q * total, total, a.K[q], a.out + (size_t)q * a.out_stride, lds);
 }
 
-dense_fn get_dense_build() { return dense_build; }
 dense_fn get_dense_prep() { return dense_prep; }
-// STREAM: non-temporal tile loads, for matrices larger than the last-level
-// (MALL) cache; smaller ones are read with normal loads and stay resident
-// between batches.
-dense_fn get_dense_scores(int NT, bool stream) {
-    switch (NT) {
-        case 1: return stream ? dense_scores<1, true> : dense_scores<1, false>;
-        case 2: return stream ? dense_scores<2, true> : dense_scores<2, false>;
-        case 4: return stream ? dense_scores<4, true> : dense_scores<4, false>;
-        default: return nullptr;
-    }
-}
 dense_fn get_dense_select() { return dense_select; }
 dense_fn get_batch_topk_radix(int V) {
     switch (V) {

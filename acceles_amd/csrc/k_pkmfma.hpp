@@ -1,0 +1,413 @@
+// k_pkmfma.hpp -- batched INT32 stage-1 scores on the matrix cores, read from
+// the sliced packets (included by the k_pkmfma_*.hip instantiation units).
+//
+// The scores are exact integer dot products of the quantized rows with the
+// int8 queries (quantify_util.py:5-6, 173-196), computed for a block of
+// queries per pass over the matrix with v_mfma_i32_32x32x32_i8.  The matrix
+// is read once per pass as its packets (about 2 bytes per non-zero); a
+// workgroup expands one 64-row slice at a time into an LDS image with one
+// byte per (row, column), and each wavefront holds the B operands of its
+// query tiles in registers for the whole kernel.  Only the four best
+// (score, row) keys of each (query, slice) leave the kernel.
+//
+// The wavefronts of a workgroup have two roles, one of each per SIMD.  MFMA
+// wavefronts run a straight-line chain per slice: while it reads slice i's
+// image, the same code reduces the accumulators of slice i-1 to their
+// four-key lists.  Decode wavefronts meanwhile fill the other image with the
+// background value and expand slice i+1's packets into it, with the packets
+// of slice i+2 in flight.  So the matrix cores, the vector ALUs and the
+// memory work at the same time.
+#pragma once
+
+#include "kernels_common.hpp"
+
+#include <utility>
+
+typedef int pv4i __attribute__((ext_vector_type(4)));
+typedef int pv16i __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint32_t pk_med3(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t d;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
+__device__ __forceinline__ int32_t pk_med3s(int32_t a, int32_t b, int32_t c) {
+    int32_t d;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
+// keeps k0 >= k1 >= k2 >= k3 the four largest keys offered: the new i-th
+// entry is the median of the old (i-1)-th, i-th and x
+__device__ __forceinline__ void pk_top4(uint32_t &k0, uint32_t &k1, uint32_t &k2, uint32_t &k3, uint32_t x) {
+    const uint32_t n3 = pk_med3(k2, k3, x), n2 = pk_med3(k1, k2, x), n1 = pk_med3(k0, k1, x);
+    k0 = max(k0, x);
+    k1 = n1;
+    k2 = n2;
+    k3 = n3;
+}
+
+// The same over signed keys.
+__device__ __forceinline__ void pk_top4s(int32_t &k0, int32_t &k1, int32_t &k2, int32_t &k3, int32_t x) {
+    const int32_t n3 = pk_med3s(k2, k3, x), n2 = pk_med3s(k1, k2, x), n1 = pk_med3s(k0, k1, x);
+    k0 = max(k0, x);
+    k1 = n1;
+    k2 = n2;
+    k3 = n3;
+}
+
+// Pre-key of accumulator value rr of row tile rt (a lane's rows rt*32 +
+// (rr/4)*8 + rr%4, plus 4 in the upper half-wave): (score << 6) | row, signed
+// (|score| < 2^25); the row part is an inline constant.
+__device__ __forceinline__ int32_t pk_prekey(int32_t v, int rt, int rr) {
+    return (int32_t)(((uint32_t)v << 6) | (uint32_t)(rt * 32 + (rr >> 2) * 8 + (rr & 3)));
+}
+
+// Workgroup barrier for LDS only: waits for this wavefront's LDS operations,
+// not for its global loads (a full barrier would also wait for the packet
+// loads issued for the slices ahead).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Value modes of the image bytes: the value field as is (unsigned below 8
+// bits, and signed 8-bit whose field is the two's-complement byte), unsigned
+// 8-bit stored as v - 128 (the image background is -128 and 128 * sum(query)
+// is added back per query), and signed below 8 bits (sign-extended).
+enum { PK_PLAIN = 0, PK_U8 = 1, PK_SIGNED = 2 };
+
+// Entry j (0..31) of a packet group (E dwords of one lane = 32 entries of its
+// row) stored into the row's image line (row_base = row * stride): the byte
+// of column c at row_base + c.  A padding entry (all bits zero) goes to the
+// line's byte `cols_pad`, which no MFMA operand reads.  With DUP (a row may
+// name a column twice) the value is added to the byte instead (the host
+// checked that every such sum fits it); a padding entry then adds 0.
+template <int E, int VM, bool DUP>
+__device__ __forceinline__ void pk_entry(const uint32_t (&w)[E], int j, unsigned char *img, uint32_t row_base,
+                                         uint32_t cols_pad, int cb, int vb, uint32_t cmask) {
+    const int pos = j * E, wi = pos >> 5, sh = pos & 31;
+    const uint32_t hi = (wi + 1 < E) ? w[wi + 1] : 0u;
+    const uint32_t bits = (sh + E <= 32) ? (w[wi] >> sh) : __builtin_amdgcn_alignbit(hi, w[wi], sh);
+    const uint32_t c = bits & cmask;
+    uint32_t byte;
+    if (VM == PK_SIGNED) byte = (uint32_t)__builtin_amdgcn_sbfe(bits, cb, vb);
+    else byte = __builtin_amdgcn_ubfe(bits, cb, vb);
+    if (DUP) {
+        unsigned char *p = img + row_base + c;
+        *p = (unsigned char)(*p + byte);
+    } else {
+        if (VM == PK_U8) byte ^= 0x80u;
+        img[row_base + (bits > cmask ? c : cols_pad)] = (unsigned char)byte;
+    }
+}
+
+// Wavefront roles: up to kPkMfmaWaves MFMA wavefronts (one 32-query tile
+// each, its B operands in registers) and kPkDecodeWaves decode wavefronts
+// (packets into the LDS images), two wavefronts per SIMD: while one SIMD's
+// MFMA wavefront multiplies, its decode wavefront expands the next slice.
+struct PkShape {
+    static constexpr int kThreads = (kPkMfmaWaves + kPkDecodeWaves) * 64;
+    static constexpr int kWavesPerEU = 2;
+};
+
+// Registers of an MFMA wavefront (one object, so that the compile-time steps
+// below reach them; every index is a constant).  Rounds alternate between
+// parity 0 and 1: round P accumulates slice s into acc[P] while it reduces
+// acc[1-P] (slice s - 1) to the four-key lists.
+struct PkMRegs {
+    pv4i ar[4][2];    // A operands in flight (ring)
+    pv16i acc[2][2];  // [round parity][row tile]
+    int32_t kk[4];    // four-pre-key list of the previous slice
+};
+
+// What every round of pk_scores uses.
+struct PkCtx {
+    const DenseArgs *a;
+    unsigned char *smem;
+    int lane, hi, wave;
+    int64_t end;  // one past this workgroup's last slice
+    uint32_t cols_pad, stride, img_bytes, fill, cmask, row_base, a_base;
+    int cb, vb;
+    __amdgpu_buffer_rsrc_t meta;  // a.slices
+};
+
+// Step kb of an MFMA round of parity P and the list work scheduled with it:
+// a compile-time step, so the B operands and accumulators are addressed by
+// constants.
+template <int KB, int P, bool LISTS, int kb>
+__device__ __forceinline__ void pk_mstep(PkMRegs &r, const pv4i (&B)[KB], const pv4i *A0, const pv4i *A1) {
+    constexpr int RING = 4, DIST = 2;  // A operands: ring slots, column blocks ahead
+    constexpr int ZS = KB / 4;         // the decode wavefronts' fill barrier
+    if constexpr (kb + DIST < KB) {    // constant offsets; blocks past nkb meet zero B operands
+        r.ar[(kb + DIST) % RING][0] = A0[(kb + DIST) * 2];
+        r.ar[(kb + DIST) % RING][1] = A1[(kb + DIST) * 2];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    r.acc[P][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(r.ar[kb % RING][0], B[kb], r.acc[P][0], 0, 0, 0);
+    r.acc[P][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(r.ar[kb % RING][1], B[kb], r.acc[P][1], 0, 0, 0);
+    if constexpr (kb == ZS) lds_barrier();
+    // the previous slice's accumulators into the list, value v at step v*KB/32
+#pragma unroll
+    for (int v = 0; v < 32; ++v)
+        if (LISTS && v * KB / 32 == kb)
+            pk_top4s(r.kk[0], r.kk[1], r.kk[2], r.kk[3], pk_prekey(r.acc[1 - P][v / 16][v & 15], v / 16, v & 15));
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int KB, int P, bool LISTS, int... KBS>
+__device__ __forceinline__ void pk_mchain(std::integer_sequence<int, KBS...>, PkMRegs &r, const pv4i (&B)[KB],
+                                          const pv4i *A0, const pv4i *A1) {
+    (pk_mstep<KB, P, LISTS, KBS>(r, B, A0, A1), ...);
+}
+
+// The four-key list of this wavefront's tile (each lane's pre-keys k0..k3,
+// taken modulo 2^32): offset to the query's keys ((score + corr + 2^25) << 6
+// | row, unsigned), both halves of the lane pair merged, stored for slice sx.
+__device__ __forceinline__ void pk_store_list(const PkCtx &c, uint32_t Dk, int q, uint32_t k0, uint32_t k1,
+                                              uint32_t k2, uint32_t k3, int64_t sx) {
+    k0 += Dk;
+    k1 += Dk;
+    k2 += Dk;
+    k3 += Dk;
+    const uint32_t o0 = (uint32_t)__shfl_xor((int)k0, 32), o1 = (uint32_t)__shfl_xor((int)k1, 32);
+    const uint32_t o2 = (uint32_t)__shfl_xor((int)k2, 32), o3 = (uint32_t)__shfl_xor((int)k3, 32);
+    pk_top4(k0, k1, k2, k3, o0);
+    pk_top4(k0, k1, k2, k3, o1);
+    pk_top4(k0, k1, k2, k3, o2);
+    pk_top4(k0, k1, k2, k3, o3);
+    const DenseArgs &a = *c.a;
+    if (c.hi == 0 && q < a.nq) *(uint4 *)(a.tk + ((size_t)q * a.nslices + sx) * 4) = make_uint4(k0, k1, k2, k3);
+}
+
+// MFMA round of parity P: slice s (image P) multiplied into acc[P]; the list
+// of the previous slice (acc[1-P], none for the first slice: !LISTS)
+// reduced during the chain and stored.
+template <int KB, int P, bool LISTS>
+__device__ __forceinline__ void pk_mround(const PkCtx &c, PkMRegs &r, const pv4i (&B)[KB], uint32_t Dk, int q,
+                                          int64_t s_prev) {
+    const unsigned char *cur = c.smem + (P ? c.img_bytes : 0u);
+    r.acc[P][0] = (pv16i){0};
+    r.acc[P][1] = (pv16i){0};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) r.kk[e] = INT32_MIN;
+    const pv4i *A0 = (const pv4i *)(cur + c.a_base);
+    const pv4i *A1 = (const pv4i *)(cur + c.a_base + 32u * c.stride);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        r.ar[j][0] = A0[j * 2];
+        r.ar[j][1] = A1[j * 2];
+    }
+    pk_mchain<KB, P, LISTS>(std::make_integer_sequence<int, KB>(), r, B, A0, A1);
+    if constexpr (LISTS) pk_store_list(c, Dk, q, r.kk[0], r.kk[1], r.kk[2], r.kk[3], s_prev);
+    lds_barrier();  // this slice's image read, the next one written
+}
+
+// The list of the last slice (accumulators of parity P).
+template <int P>
+__device__ __forceinline__ void pk_last_list(const PkCtx &c, const PkMRegs &r, uint32_t Dk, int q, int64_t sx) {
+    int32_t k0 = INT32_MIN, k1 = INT32_MIN, k2 = INT32_MIN, k3 = INT32_MIN;
+#pragma unroll
+    for (int v = 0; v < 32; ++v) pk_top4s(k0, k1, k2, k3, pk_prekey(r.acc[P][v / 16][v & 15], v / 16, v & 15));
+    pk_store_list(c, Dk, q, k0, k1, k2, k3, sx);
+}
+
+// An MFMA wavefront: query tile `tile` of block qb over slices s .. end-1.
+template <int KB>
+__device__ __forceinline__ void pk_mmain(const PkCtx &c, int qb, int tile, int64_t s) {
+    const DenseArgs &a = *c.a;
+    // B operands of the tile (zero past nkb)
+    pv4i B[KB];
+    {
+        const pv4i *q = (const pv4i *)a.qtab + (size_t)qb * a.nkb * a.NT * 64 + c.lane;
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+            B[kb] = kb < a.nkb ? q[((size_t)kb * a.NT + tile) * 64] : (pv4i){0, 0, 0, 0};
+    }
+    // the lane's query and its key offset D = ((corr + 2^25) << 6) + 4*hi
+    const int q = qb * a.NT * 32 + tile * 32 + (c.lane & 31);
+    const uint32_t Dk = ((uint32_t)(a.corr[q] + (1 << 25)) << 6) + (uint32_t)(c.hi * 4);
+    PkMRegs r;
+    lds_barrier();  // both images filled
+    lds_barrier();  // the first slice decoded
+    pk_mround<KB, 0, false>(c, r, B, Dk, q, -1);
+    int64_t s_prev = s;
+    ++s;
+    for (;;) {
+        if (s >= c.end) {
+            pk_last_list<0>(c, r, Dk, q, s_prev);
+            return;
+        }
+        pk_mround<KB, 1, true>(c, r, B, Dk, q, s_prev);
+        s_prev = s++;
+        if (s >= c.end) {
+            pk_last_list<1>(c, r, Dk, q, s_prev);
+            return;
+        }
+        pk_mround<KB, 0, true>(c, r, B, Dk, q, s_prev);
+        s_prev = s++;
+    }
+}
+
+// Packet groups g = first, first + kPkDecodeWaves, ... of slice sl into image
+// p (DUPV: a row may name a column twice; then one wavefront decodes every
+// group, so that the additions to a byte are ordered).
+template <int E, int VM, bool DUPV>
+__device__ __forceinline__ void pk_decode_groups(const PkCtx &c, uint2 sl, int first, int step, unsigned char *p) {
+    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(c.a->words, sl);
+    for (int g0 = first * E; g0 < (int)sl.y; g0 += step * E) {
+        uint32_t w[E];
+        load_group<E>(w, rs, g0);
+#pragma unroll
+        for (int j = 0; j < 32; ++j) pk_entry<E, VM, DUPV>(w, j, p, c.row_base, c.cols_pad, c.cb, c.vb, c.cmask);
+    }
+}
+
+// The background into image p, by the decode wavefronts' lanes dl.
+__device__ __forceinline__ void pk_fill(const PkCtx &c, unsigned char *p, uint32_t n16, uint32_t dl) {
+    pv4i *p4 = (pv4i *)p;
+    const pv4i f = {(int)c.fill, (int)c.fill, (int)c.fill, (int)c.fill};
+    for (uint32_t i = dl; i < n16; i += kPkDecodeWaves * 64) p4[i] = f;
+}
+
+// Metadata of slice min(s, end - 1) as a vector load: a scalar load would be
+// waited for at the next LDS barrier (both count in lgkmcnt), exposing its
+// latency once per round.
+__device__ __forceinline__ uint2 pk_slice_meta(const PkCtx &c, int64_t s) {
+    const uint32_t i = (uint32_t)min(s, c.end - 1);
+    uint2 m;
+    m.x = __builtin_amdgcn_raw_buffer_load_b32(c.meta, (int)(i * 8u), 0, 0);
+    m.y = __builtin_amdgcn_raw_buffer_load_b32(c.meta, (int)(i * 8u + 4u), 0, 0);
+    return m;
+}
+
+// Decode round of parity P (the MFMA wavefronts multiply slice s from image
+// P): image 1-P filled, then slice s + 1 (metadata sl1) decoded into it --
+// group dw from the registers w[P], requested a round earlier, and groups dw
+// + kPkDecodeWaves, ... loaded here; group dw of slice s + 2 (sl2)
+// requested into w[1-P].  sl1, sl2 advance by one slice.
+template <int E, int VM, bool DEC, int P>
+__device__ __forceinline__ void pk_dround(const PkCtx &c, uint32_t (&w)[2][E], int dw, int64_t s, uint2 &sl1,
+                                          uint2 &sl2) {
+    const DenseArgs &a = *c.a;
+    unsigned char *nxt = c.smem + (P ? 0u : c.img_bytes);
+    const int64_t s3 = s + 3;
+    const uint2 s3m = pk_slice_meta(c, s3);
+    if constexpr (DEC) load_group<E>(w[1 - P], slice_rsrc(a.words, sl2), dw * E);
+    pk_fill(c, nxt, c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
+    lds_barrier();  // the image holds the background
+    if constexpr (DEC) {
+        if (dw * E < (int)sl1.y) {
+#pragma unroll
+            for (int j = 0; j < 32; ++j) pk_entry<E, VM, false>(w[P], j, nxt, c.row_base, c.cols_pad, c.cb, c.vb, c.cmask);
+            if ((dw + kPkDecodeWaves) * E < (int)sl1.y)
+                pk_decode_groups<E, VM, false>(c, sl1, dw + kPkDecodeWaves, kPkDecodeWaves, nxt);
+        }
+    } else if (dw == 0) {
+        pk_decode_groups<E, VM, true>(c, sl1, 0, 1, nxt);
+    }
+    sl1 = sl2;
+    sl2 = s3 < c.end ? s3m : make_uint2(0u, 0u);
+    lds_barrier();  // the next slice's image complete
+}
+
+// A decode wavefront (dw = 0 .. kPkDecodeWaves-1) over slices s .. end-1
+// (loads past a slice's end return zeros and fetch nothing; slices past the
+// matrix read as empty).
+template <int E, int VM, bool DEC>
+__device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
+    const DenseArgs &a = *c.a;
+    uint32_t w[2][E];
+    pk_fill(c, c.smem, 2u * c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
+    const uint2 s0m = pk_slice_meta(c, s);
+    const uint2 s1m = pk_slice_meta(c, s + 1), s2m = pk_slice_meta(c, s + 2);
+    uint2 sl1 = s + 1 < c.end ? s1m : make_uint2(0u, 0u);
+    uint2 sl2 = s + 2 < c.end ? s2m : make_uint2(0u, 0u);
+    if constexpr (DEC) load_group<E>(w[0], slice_rsrc(a.words, sl1), dw * E);
+    lds_barrier();  // both images filled
+    if constexpr (DEC) pk_decode_groups<E, VM, false>(c, s0m, dw, kPkDecodeWaves, c.smem);
+    else if (dw == 0) pk_decode_groups<E, VM, true>(c, s0m, 0, 1, c.smem);
+    lds_barrier();  // the first slice decoded
+    pk_dround<E, VM, DEC, 0>(c, w, dw, s, sl1, sl2);
+    ++s;
+    for (;;) {
+        if (s >= c.end) return;
+        pk_dround<E, VM, DEC, 1>(c, w, dw, s, sl1, sl2);
+        if (++s >= c.end) return;
+        pk_dround<E, VM, DEC, 0>(c, w, dw, s, sl1, sl2);
+        ++s;
+    }
+}
+
+// Scores of the 64-row slices against query block qb (QB = 32 * NT queries,
+// NT = a.NT MFMA wavefronts, one tile each, followed by kPkDecodeWaves decode
+// wavefronts): grid = nwx * nqb workgroups; the nqb blocks of one slice range
+// wx get workgroup ids 8 apart, hence one XCD, and the second and later
+// blocks find the packets in its L2.  Workgroup wx takes the slices wx*per
+// .. wx*per + per-1 (per = ceil(nslices / nwx)): consecutive slices, so that
+// a query's four-key lists of one workgroup fill whole cache lines in L2
+// before they are written back.  KB: column blocks the B registers hold (nkb <= KB; the chain runs over
+// all KB with zero B operands past nkb, so that it has no control flow).
+// Every wavefront passes the same barriers: two before the first round, two
+// per round (the decode wavefronts' fill, then the end of the round).
+//
+// Image of a slice (LDS): row r's bytes at r * stride + column, stride =
+// cols_pad + 16; the 16 bytes past cols_pad take the padding entries.  The A
+// operand (row tile rt, column block kb) of lane l is the 16 bytes of row
+// rt*32 + l%32 at column kb*32 + (l/32)*16: with stride/4 = 8m + 4, any 8
+// consecutive lanes read disjoint banks.  The query table pairs each B byte
+// with the same column.
+template <int E, int VM, int KB>
+__global__ __launch_bounds__(PkShape::kThreads)
+__attribute__((amdgpu_waves_per_eu(PkShape::kWavesPerEU, PkShape::kWavesPerEU))) void pk_scores(DenseArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wave = (int)(threadIdx.x >> 6), lane = lane_id(), hi = lane >> 5;
+    const int nqb = a.nqb;
+    const int c8 = (int)blockIdx.x % (8 * nqb);
+    const int qb = c8 / 8;
+    const int wx = ((int)blockIdx.x / (8 * nqb)) * 8 + (c8 % 8), nwx = (int)gridDim.x / nqb;
+    const int64_t per = (a.nslices + nwx - 1) / nwx, s0 = (int64_t)wx * per;
+    if (s0 >= a.nslices) return;  // the whole workgroup
+    PkCtx c;
+    c.a = &a;
+    c.smem = smem;
+    c.lane = lane;
+    c.hi = hi;
+    c.end = min(s0 + per, a.nslices);
+    c.wave = wave;
+    c.cols_pad = (uint32_t)a.cols_pad;
+    c.stride = c.cols_pad + 16u;
+    c.img_bytes = 64u * c.stride;
+    c.fill = VM == PK_U8 ? 0x80808080u : 0u;
+    c.cmask = (1u << a.col_bits) - 1u;
+    c.row_base = (uint32_t)lane * c.stride;                            // scatter: lane = row
+    c.a_base = (uint32_t)(lane & 31) * c.stride + (uint32_t)hi * 16u;  // A operand of row tile 0
+    c.cb = a.col_bits;
+    c.vb = a.val_bits;
+    c.meta = __builtin_amdgcn_make_buffer_rsrc((void *)a.slices, (short)0, (int)(a.nslices * 8),
+                                               0x00020000 /* 32-bit data format */);
+    if (wave < a.NT) pk_mmain<KB>(c, qb, wave, s0);
+    else if (a.dup) pk_dmain<E, VM, false>(c, wave - a.NT, s0);
+    else pk_dmain<E, VM, true>(c, wave - a.NT, s0);
+}
+
+#define PK_CASES_VM(VM, KB)                    \
+    switch (E) {                               \
+        case 10: return pk_scores<10, VM, KB>; \
+        case 11: return pk_scores<11, VM, KB>; \
+        case 12: return pk_scores<12, VM, KB>; \
+        case 13: return pk_scores<13, VM, KB>; \
+        case 14: return pk_scores<14, VM, KB>; \
+        case 15: return pk_scores<15, VM, KB>; \
+        case 16: return pk_scores<16, VM, KB>; \
+        case 17: return pk_scores<17, VM, KB>; \
+        case 18: return pk_scores<18, VM, KB>; \
+        default: return nullptr;               \
+    }
+
+#define PK_CASES(KB)                                    \
+    if (vm == PK_U8) { PK_CASES_VM(PK_U8, KB) }         \
+    if (vm == PK_SIGNED) { PK_CASES_VM(PK_SIGNED, KB) } \
+    PK_CASES_VM(PK_PLAIN, KB)

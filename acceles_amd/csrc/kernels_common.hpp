@@ -593,39 +593,47 @@ struct QTable {
 
 
 // ---------------------------------------------------------------------------
-// batched low-bit path on dense int8 tiles (matrix cores)
+// batched low-bit path on the matrix cores, read from the packets
 // ---------------------------------------------------------------------------
-//   dense_build    sliced packets -> dense int8 tiles, once per matrix;
 //   dense_prep     quantizes the batch's queries into the B-operand table;
-//   dense_scores   v_mfma_i32_32x32x32_i8 over each 64-row slice and a block
-//                  of 32*NT queries; per (query, slice) it keeps only the four
-//                  best (score, row-in-slice) keys;
+//   pk_scores      v_mfma_i32_32x32x32_i8 over each 64-row slice and a block
+//                  of queries: decode wavefronts expand the slice's packets
+//                  into an LDS image of the A operands, each MFMA wavefront
+//                  holds its query tile's B operands in registers; per (query,
+//                  slice) only the four best (score, row-in-slice) keys are
+//                  kept;
 //   dense_select   per (partition, query): threshold = the M-th best slice
-//                  maximum, candidates from the four-key lists, exact rescan of
-//                  any slice whose list may have dropped a candidate, Top-M,
-//                  and the FP32 rescoring of those M rows;
+//                  maximum, candidates from the four-key lists, exact rescan
+//                  (from the packets) of any slice whose list may have dropped
+//                  a candidate, Top-M, and the FP32 rescoring of those M rows;
 //   batch_topk     per query: its Top-K[q] of the rescored keys.
 //
-// Tile layout: tile (rt, kb) of slice s (rows s*64 + rt*32 .. +31, columns
-// kb*32 .. +31) is 1 KB at ((kb*nslices + s)*2 + rt)*1024 (column-block
-// major, so concurrent wavefronts stream neighbouring memory); lane l's 16 bytes hold
-// row l%32, columns kb*32 + (l/32)*16 + 0..15 -- the A-operand register of
-// the MFMA.  The query table uses the same lane mapping for the B operand
-// (query l%32 of a 32-query tile), so every k is paired identically in both.
-// Unsigned 8-bit values are stored as v-128 (absent entries as -128) and
-// 128 * sum(query) is added back per query; other widths are stored as is.
+// Image of a slice (LDS): row r of the slice at r * (cols_pad + 16), one
+// byte per column; the 16 bytes past cols_pad take the padding entries.  The
+// A operand (row tile rt, column block kb) of lane l is the 16 bytes of row
+// rt*32 + l%32 at column kb*32 + (l/32)*16.  The query table uses the same
+// lane mapping for the B operand (query l%32 of a 32-query tile), so every k
+// is paired identically in both.  Unsigned 8-bit values are stored as v-128
+// (absent entries as -128) and 128 * sum(query) is added back per query;
+// other widths are stored as is.  Two images, then kPkLdsTail bytes that the
+// chain's reads of column blocks past nkb (multiplied by zero B operands)
+// may reach.
+static constexpr int kPkLdsTail = 34 * 32;
+// pk_scores workgroups: up to kPkMfmaWaves MFMA wavefronts (one 32-query tile
+// each) and kPkDecodeWaves decode wavefronts.
+static constexpr int kPkMfmaWaves = 4;
+static constexpr int kPkDecodeWaves = 4;
 
 struct DenseArgs {
-    const uint32_t *words;       // sliced packets (dense_build, exact rescans)
+    const uint32_t *words;       // sliced packets
     const uint2 *slices;
     const int32_t *perm;
     const int64_t *part_slice;   // split + 1
     const uint8_t *nvalid;       // valid rows per slice (a prefix of its lanes)
-    int8_t *tiles;
     int64_t nslices;
-    int cols, cols_pad, nkb, col_bits, val_bits, is_signed, u8off;
+    int cols, cols_pad, nkb, col_bits, val_bits, is_signed, u8off, dup;
     const float *X;              // nq x cols FP32 queries
-    int nq, NT, nqb;
+    int nq, NT, nqb;             // queries; 32-query tiles per block; blocks
     int8_t *qtab;                // nqb x nkb x NT x 1024 B-operand bytes
     int8_t *qplain;              // nq x cols_pad quantized queries
     int32_t *corr;               // nqb*32*NT per-query offsets
@@ -673,9 +681,10 @@ batch_fn get_batch_scores32(int E, bool is_signed, int bits);  // k_batch32.hip
 batch_fn get_batch_select();                                   // k_batch.hip
 batch_fn get_batch_final(int R);
 typedef void (*dense_fn)(DenseArgs);
-dense_fn get_dense_build();                                    // k_dense.hip
-dense_fn get_dense_prep();
-dense_fn get_dense_scores(int NT, bool stream);
+dense_fn get_dense_prep();                                     // k_dense.hip
 dense_fn get_dense_select();
 dense_fn get_batch_topk_radix(int V);
 dense_fn get_batch_topk(int R);
+dense_fn get_pk_scores_8(int E, int vm);                       // k_pkmfma_a.hip
+dense_fn get_pk_scores_16(int E, int vm);                      // k_pkmfma_b.hip
+dense_fn get_pk_scores_32(int E, int vm);                      // k_pkmfma_c.hip

@@ -289,19 +289,12 @@ extern "C" int acc_server_add_matrix(acc_server *s, const acc_csr *csr, const ac
     acc_gpu_matrix *m = nullptr;
     int rc = acc_gpu_upload(csr, pk, s->o.device, &m);
     if (rc != ACC_OK) return release(rc);
-    // the resident matrix plus the workspace of a full batch and of a small
-    // one (dense tiles, if the matrix can use them, are built here)
+    // the resident matrix plus the workspace of a full batch and of a small one
     acc_gpu_info_t info;
     acc_gpu_matrix_info(m, &info);
     if (info.device_bytes > have) {
         acc_gpu_matrix_destroy(m);
         return release(fail(ACC_ERR_BUDGET, "matrix does not fit in the server's memory budget"));
-    }
-    if (info.dense_ok) {
-        const int64_t nsl = (info.rows + 63) / 64 + info.split;
-        const int64_t dense = nsl * 2 * (((int64_t)info.cols + 31) / 32) * 1024;
-        if (info.device_bytes + dense > have) rc = acc_gpu_set_batch_mode(m, ACC_BATCH_SPARSE, 0);
-        else rc = acc_gpu_set_batch_mode(m, ACC_BATCH_AUTO, 0);
     }
     const int warm[2] = {std::min(15, s->o.max_batch), s->o.max_batch};
     float *x = nullptr;

@@ -491,13 +491,13 @@ def main():
     nslots = pk.info.nslices * 64
     dense = len(bt_ms) == 4
     if dense:
-        # dense int8 tiles on the matrix cores: prep, scores, select + FP32 rescoring, final
+        # matrix cores over the packets: prep, scores, select + FP32 rescoring, final
         names = ["prep", "scores", "select_rescore", "final"]
         t_sc = bt_ms[1]
         cols_pad = (a.cols + 31) // 32 * 32
-        bs_bytes = g.info.dense_bytes + B * cols_pad + B * pk.info.nslices * 16
+        bs_bytes = pk.info.stream_bytes + B * a.cols * 4
         bs_ops = 2.0 * nslots * cols_pad * ((B + 31) // 32 * 32)
-        sc_kernel = "dense_scores"
+        sc_kernel = "pk_scores"
     else:
         names = ["scores", "select", "final"]
         t_sc = bt_ms[0]

@@ -254,8 +254,8 @@ typedef struct {
     int64_t stream_bytes;  /* low-bit bytes read per stage-1 pass            */
     int64_t device_bytes;  /* total device memory held by the handle         */
     int32_t batch_mode;    /* ACC_BATCH_* in effect                          */
-    int32_t dense_ok;      /* the dense int8 tile layout can hold the matrix */
-    int64_t dense_bytes;   /* bytes of the dense tiles (0 until built)       */
+    int32_t dense_ok;      /* ACC_BATCH_DENSE can score the matrix exactly   */
+    int64_t dense_bytes;   /* device bytes held for it beyond the packets: 0 */
 } acc_gpu_info_t;
 
 int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info);
@@ -267,7 +267,7 @@ int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info);
  * kernel's own start and end); acc_gpu_stage_times waits for the last query
  * and returns up to n kernel durations in ms (low-bit: slice maxima, select
  * + rescoring, final Top-K; batched packets: scores, select, final; batched
- * dense tiles: prep, scores, select + rescoring, final; FP32: SpMV, select,
+ * matrix cores: prep, scores, select + rescoring, final; FP32: SpMV, select,
  * final select) and their count. */
 int acc_gpu_set_profiling(acc_gpu_matrix *m, int enable);
 int acc_gpu_stage_times(acc_gpu_matrix *m, float *ms, int n, int *count);
@@ -311,18 +311,18 @@ int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq,
                               int out_stride, void *stream);
 
 /* Scoring layout of the batched path.  The reference scores one query per
- * pass (host_spmv_ucsr.cpp:1084-1180); a batch here is one pass per block of
- * queries, either over the sliced packets (v_dot4 on the vector ALUs) or over
- * dense int8 tiles on the matrix cores (v_mfma_i32_32x32x32_i8; 64-row slices
- * x columns rounded up to 32, one byte per entry).  Both give the same
- * integer scores and the same results.  ACC_BATCH_DENSE fails with a message
- * when the layout cannot hold the matrix exactly (a row whose values in one
- * column sum past one byte, |score| >= 2^25, or more than 4960 columns);
- * ACC_BATCH_AUTO uses the dense tiles when they can hold the matrix and the
- * batch has at least
- * min_dense_batch queries (0 keeps the current value; default 16), building
- * them on first use.  Setting DENSE or AUTO on an eligible matrix builds the
- * tiles now. */
+ * pass (host_spmv_ucsr.cpp:1084-1180); a batch here is one pass over the
+ * sliced packets per block of queries, scored either on the vector ALUs
+ * (ACC_BATCH_SPARSE, v_dot4) or on the matrix cores (ACC_BATCH_DENSE,
+ * v_mfma_i32_32x32x32_i8: each 64-row slice is expanded from its packets
+ * into an int8 image in LDS, one byte per (row, column), inside the scoring
+ * kernel; nothing beyond the packets is stored).  Both give the same integer
+ * scores and the same results.  ACC_BATCH_DENSE fails with a message when
+ * the matrix cores cannot score the matrix exactly (a row whose values in one
+ * column sum past one byte, |score| >= 2^25, more than 1024 columns, or
+ * entries wider than 18 bits); ACC_BATCH_AUTO uses them when they can and the
+ * batch has at least min_dense_batch queries (0 keeps the current value;
+ * default 16). */
 #define ACC_BATCH_AUTO 0
 #define ACC_BATCH_SPARSE 1
 #define ACC_BATCH_DENSE 2
@@ -333,9 +333,8 @@ int acc_gpu_set_batch_mode(acc_gpu_matrix *m, int mode, int min_dense_batch);
  * partition count and CK are fixed at build time, types.hpp:54,81): times one
  * query per pass, the packet batch and the dense batch (where the matrix
  * allows it) on synthetic queries at batch sizes 1, 2, 4, ... up to max_batch,
- * and keeps the fastest layout per batch size in the handle (builds the dense
- * tiles unless the handle was set to ACC_BATCH_SPARSE or _SINGLE, which
- * keeps the dense layout out).  points[i] receives each measured size (up to cap entries; *count
+ * and keeps the fastest layout per batch size in the handle (a handle set to
+ * ACC_BATCH_SPARSE or _SINGLE keeps the matrix cores out).  points[i] receives each measured size (up to cap entries; *count
  * gets the number written); times are per batch in microseconds, -1 where a
  * layout does not apply.  Results do not depend on the layout. */
 typedef struct {

@@ -243,15 +243,21 @@ def test_lowbit_batch_uneven_k(acc, gpu, mode):
     (20000, 500, (40, 100), 8, 32, False, 40),   # unsigned 8-bit (offset bytes), 64-query blocks
     (20000, 900, (30, 90), 8, 16, True, 37),     # signed 8-bit
     (20000, 500, (40, 100), 4, 32, False, 9),    # 4-bit, one 32-query block
-    (30000, 1024, (0.5, 40.0), 8, 64, False, 33),  # power-law rows, 1024 columns
-    (12000, 300, (5, 30), 4, 8, False, 300),     # 300 queries: three 128-query blocks, a ragged last one
-    (40000, 4096, (0, 300), 4, 16, False, 24),   # 4096 columns (4-bit: scores stay below 2^25 for the tiles): query columns past what the kernels hold in registers
+    (30000, 1024, (0.5, 40.0), 8, 64, False, 33),  # power-law rows, 1024 columns (32 column blocks)
+    (20000, 900, (30, 90), 8, 32, False, 256),   # 900 columns, 256 queries: two 128-query blocks
+    (20000, 500, (40, 100), 8, 32, False, 256),  # 256 queries in one block of eight wavefronts
+    (12000, 300, (5, 30), 4, 8, False, 300),     # 300 queries: two blocks of five 32-query tiles, a ragged last one
+    (40000, 4096, (0, 300), 4, 16, False, 24),   # 4096 columns: query columns past what the kernels hold in registers
     (50000, 200, (10, 30), 8, 2, False, 20),     # partitions of 391 slices (threshold by extraction)
     (150000, 200, (10, 30), 8, 2, False, 20),    # partitions of 1172 slices (threshold by collectors)
 ])
 def test_lowbit_batch_matches_oracle(acc, gpu, mode, rows, cols, deg, bits, split, signed, nq):
     kind = 1 if isinstance(deg[0], float) else 0
     csr, pk, g = make(acc, rows, cols, deg, bits, split, signed, kind, seed=rows + nq)
+    if mode == 2 and cols > 1024:  # the matrix cores hold B registers for up to 1024 columns
+        with pytest.raises(acc.AccelesError, match="more than 1024 columns"):
+            g.set_batch_mode(mode)
+        mode = 0  # AUTO: the packet batch
     g.set_batch_mode(mode)
     X = acc.generate_queries(nq, cols, seed=nq)
     ck = min(64, 512 // split) * split
