@@ -163,6 +163,12 @@ __device__ __forceinline__ void pk_mchain(std::integer_sequence<int, KBS...>, Pk
     (pk_mstep<KB, P, LISTS, KBS>(r, B, A0, A1), ...);
 }
 
+// Lanes 0..31 receive the value of lane + 32 (v_permlane32_swap; lanes
+// 32..63 keep their own).
+__device__ __forceinline__ uint32_t pk_upper_half(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_permlane32_swap((int)v, (int)v, false, false)[1];
+}
+
 // The four-key list of this wavefront's tile (each lane's pre-keys k0..k3,
 // taken modulo 2^32): offset to the query's keys ((score + corr + 2^25) << 6
 // | row, unsigned), both halves of the lane pair merged, stored for slice sx.
@@ -172,8 +178,8 @@ __device__ __forceinline__ void pk_store_list(const PkCtx &c, uint32_t Dk, int q
     k1 += Dk;
     k2 += Dk;
     k3 += Dk;
-    const uint32_t o0 = (uint32_t)__shfl_xor((int)k0, 32), o1 = (uint32_t)__shfl_xor((int)k1, 32);
-    const uint32_t o2 = (uint32_t)__shfl_xor((int)k2, 32), o3 = (uint32_t)__shfl_xor((int)k3, 32);
+    // the upper half-wave's lists into lanes 0..31 (the lanes that store)
+    const uint32_t o0 = pk_upper_half(k0), o1 = pk_upper_half(k1), o2 = pk_upper_half(k2), o3 = pk_upper_half(k3);
     pk_top4(k0, k1, k2, k3, o0);
     pk_top4(k0, k1, k2, k3, o1);
     pk_top4(k0, k1, k2, k3, o2);
