@@ -48,8 +48,11 @@ def parse():
     ap.add_argument("--ck", type=int, default=512)
     ap.add_argument("--k", type=int, default=100)
     ap.add_argument("--batch", type=int, default=100)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU baseline sample")
+    ap.add_argument("--batches-per-step", type=int, default=10,
+                    help="embedding: batches of --batch queries per timed step")
+    ap.add_argument("--cpu-seconds", type=float, default=16.0, help="bound on the CPU baseline sample (four legs)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="embedding: skip the 20,000,000x1024 slice-pass roofline")
     ap.add_argument("--inflight", type=int, default=0, choices=[0, 1, 2, 3, 4],
                     help="batches (embedding, batched) or queries (large-single) in flight on as many streams; "
                          "0: 3 batches (embedding), 2 batches (batched), 2 queries (large-single)")
@@ -68,18 +71,49 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(csr, X, k, seconds):
-    """Reference CPU path (cpu_mkl_baseline.cpp:249-275 semantics: FP32 SpMV then a
-    min-heap Top-k), single thread, via the oracle's C restatement."""
+def cpu_threads():
+    """(cores visible to this process, threads the CPU legs use): the affinity mask, capped by
+    OMP_NUM_THREADS when that is set (the box's CPU share)."""
+    aff = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return aff, (min(aff, env) if env > 0 else aff)
+
+
+def cpu_baseline(csr, X, k, seconds, bits, split, ck, group):
+    """The reference CPU path on this box's host cores, via the oracle's C restatement
+    (oracle/oracle.c), each leg over as many queries as fit in seconds/4:
+      fp32_1t    FP32 CSR SpMV then a min-heap Top-k (cpu_mkl_baseline.cpp:249-275), one thread;
+      fp32_all   the same on every core: rows split across threads, per-thread heaps merged (the
+                 reference's multicore mode, cpu_mkl_baseline.cpp:195-245, with its race removed);
+      lowbit_1t / lowbit_all
+                 the low-bit two-stage path (quantify_util.py:173-196, 354-367): INT32 scores of the
+                 quantized matrix, Top-(ck/split) per partition, FP32 rescoring, Top-k.
+    The line's value is fp32_all (the CPU path on all cores)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     ptr, idx, val = csr.arrays()
-    n = 0
-    t0 = time.perf_counter()
-    while n < len(X) and time.perf_counter() - t0 < seconds:
-        O.top_k_spmv_f32(ptr, idx, val, X[n], k)
-        n += 1
-    return n / (time.perf_counter() - t0), n
+    nproc, nt = cpu_threads()
+    qv, _ = O.quantize(val, bits, False)
+    cols = X.shape[1]
+    legs = {}
+
+    def run(name, fn):
+        n, t0 = 0, time.perf_counter()
+        while n < 4 or time.perf_counter() - t0 < seconds / 4:
+            fn(X[n % len(X)])
+            n += 1
+        legs[name] = {"queries_per_s": round(n / (time.perf_counter() - t0), 3), "queries": n}
+
+    run("fp32_1t", lambda x: O.top_k_spmv_f32(ptr, idx, val, x, k))
+    run("fp32_all", lambda x: O.top_k_spmv_f32_par(ptr, idx, val, x, k, nt))
+    run("lowbit_1t", lambda x: O.two_stage_par(ptr, idx, val, qv, x, cols, group, split, ck, k, 1))
+    run("lowbit_all", lambda x: O.two_stage_par(ptr, idx, val, qv, x, cols, group, split, ck, k, nt))
+    return {"value": legs["fp32_all"]["queries_per_s"], "unit": "queries/s", "cores": nt, "nproc": nproc,
+            "kind": "port",
+            "sample": f"queries of the same workload, about {seconds / 4:.0f} s per leg, through the reference CPU "
+                      f"path semantics restated in oracle/oracle.c; value = FP32 CSR SpMV + min-heap Top-{k} "
+                      f"(cpu_mkl_baseline.cpp:249-275) on {nt} threads",
+            "legs": legs}
 
 
 def timed(stream, fn, steps, warmup):
@@ -128,6 +162,37 @@ def line_common(a, metric, value, unit, ms, dtype, workload, extra_cfg):
             "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": dtype, "data": "synthetic",
             "config": dict({"workload": workload, "parallelism": "single GPU"}, **extra_cfg)}
+
+
+def c3_slicemax(acc, stream, reps=20):
+    """The single-query slice pass at HBM scale: 20,000,000 x 1024, row nnz U[10,30], |N(0,1)|, 8-bit
+    packets, SPLIT=64 (0.93 GB of packets, beyond the 256 MB MALL).  lowbit_slicemax is timed as a burst
+    of reps back-to-back launches between two HIP events on the query stream; achieved = (packets + the
+    FP32 query) / average launch.  traffic: rocprofv3 counter bytes per launch from profiles/, if there."""
+    rows, cols, K, CK = 20_000_000, 1024, 100, 1024
+    t0 = time.perf_counter()
+    csr = acc.CSRMatrix.generate(rows, cols, deg=(10, 30), l2norm=False, seed=0xACCE1E5 + 2)
+    pk = csr.pack(bits=8, split=64)
+    g = acc.GpuMatrix(csr, pk, device=0)
+    del csr
+    x = acc.DeviceBuffer.from_numpy(acc.generate_queries(1, cols, seed=0xACCE1E5 + 3))
+    out = acc.DeviceBuffer((K, 2), np.int32)
+    g.topk_lowbit(x, K, CK, out, stream)
+    stream.synchronize()
+    ms = g.time_lowbit_stage(x, K, CK, 0, reps, stream)
+    nbytes = pk.info.stream_bytes + cols * 4
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"[bench] 20000000x1024 slice pass: {ms * 1e3:.1f}us for {nbytes / 1e9:.3f} GB = {gbs:.0f} GB/s "
+        f"(setup {time.perf_counter() - t0:.1f}s)")
+    traffic = pmc_traffic("lowbit_slicemax", rows, cols, 8)
+    res = {"bound": "hbm", "kernel": "lowbit_slicemax", "workload": f"{rows}x{cols}, row nnz U[10,30], 8-bit, "
+           "SPLIT=64, one query", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "traffic_ratio": None if traffic is None else round(traffic / nbytes, 4),
+           "algorithmic_bytes_per_launch": int(nbytes), "avg_launch_us": round(ms * 1e3, 2),
+           "bytes_per_nnz_packed": round(pk.info.bytes_per_nnz, 4)}
+    del g, pk
+    return res
 
 
 def bench_large_single(a, acc, stream):
@@ -418,8 +483,14 @@ def main():
     xs = [Xd.at(i * a.cols) for i in range(B)]
     outs1 = [out1.at(i * K * 2) for i in range(B)]
 
+    BPS = max(1, a.batches_per_step)
+
     def batch_step():
         g.topk_lowbit_batch(Xd, ks, CK, out, K, stream)
+
+    def batch_steps():
+        for _ in range(BPS):
+            batch_step()
 
     def single_step():
         for i in range(B):
@@ -430,8 +501,8 @@ def main():
             g.topk_fp32(xs[i], K, outs1[i], stream)
 
     # ---- timed region: batched low-bit two-stage, one batch at a time (eager calls) ----
-    dt1 = timed(stream, batch_step, a.steps, a.warmup)
-    qps1 = a.steps * B / dt1
+    dt1 = timed(stream, batch_steps, a.steps, a.warmup)
+    qps1 = a.steps * BPS * B / dt1
     res = out.numpy()
 
     # ---- timed region: batches in flight (HIP-graph plans with their own workspaces on as many
@@ -445,24 +516,25 @@ def main():
     it = [0]
 
     def pipe_step():
-        j = it[0] % NF
-        plans[j].launch(streams[j])
-        it[0] += 1
+        for _ in range(BPS):
+            j = it[0] % NF
+            plans[j].launch(streams[j])
+            it[0] += 1
     dt = timed_multi(streams, pipe_step, a.steps, a.warmup)
-    qps = a.steps * B / dt
+    qps = a.steps * BPS * B / dt
     ms_step = dt * 1e3 / a.steps
     pipelined_same = bool(np.array_equal(outs2[0].numpy(), res))
     for p_ in plans:
         p_.close()
     log(f"[bench] low-bit two-stage, batched: {qps:.0f} queries/s with {NF} batches in flight ({ms_step:.3f} ms per "
-        f"{B}-query step); one batch at a time {qps1:.0f} q/s; in-flight results equal eager: {pipelined_same}")
+        f"{BPS * B}-query step); one batch at a time {qps1:.0f} q/s; in-flight results equal eager: {pipelined_same}")
 
     # ---- the other batched layout, for comparison ----
     other_qps = None
     if g.info.dense_ok:
         mode = g.info.batch_mode
         g.set_batch_mode(acc.BATCH_SPARSE if mode != acc.BATCH_SPARSE else acc.BATCH_DENSE)
-        other_qps = a.steps * B / timed(stream, batch_step, a.steps, 1)
+        other_qps = a.steps * BPS * B / timed(stream, batch_steps, a.steps, 1)
         g.set_batch_mode(mode)
         log(f"[bench] batched, other layout: {other_qps:.0f} queries/s")
 
@@ -538,14 +610,13 @@ def main():
     # ---- HBM traffic from rocprof counters, when profiled for this workload ----
     traffic = pmc_traffic(sc_kernel, a.rows, a.cols, a.bits)
 
+    c3 = None if a.no_c3 else c3_slicemax(acc, stream)
+
     cpu = None
     if not a.no_cpu:
-        cq, n = cpu_baseline(csr, X, K, a.cpu_seconds)
-        cpu = {"value": round(cq, 3), "unit": "queries/s", "cores": 1, "kind": "port",
-               "sample": f"{n} queries of the same workload through the reference CPU path semantics "
-                         f"(FP32 CSR SpMV + min-heap Top-{K}, cpu_mkl_baseline.cpp:249-275) restated in "
-                         f"oracle/oracle.c, 1 thread"}
-        log(f"[bench] cpu baseline {cq:.2f} queries/s over {n} queries")
+        cpu = cpu_baseline(csr, X, K, a.cpu_seconds, a.bits, a.split, CK, g.info.group)
+        log(f"[bench] cpu baseline ({cpu['cores']} of {cpu['nproc']} cores): " +
+            ", ".join(f"{n} {v['queries_per_s']:.1f} q/s" for n, v in cpu["legs"].items()))
 
     line = {
         "metric": "queries/s + HBM GB/s, low-bit Top-100 SpMV vs FP32 CSR at recall@100>=0.99",
@@ -563,9 +634,10 @@ def main():
         "config": {
             "workload": f"low-bit two-stage Top-{K}, batched: {a.rows}x{a.cols}, row nnz U[{a.deg[0]},{a.deg[1]}], "
                         f"|N(0,1)| L2-normalised, {a.bits}-bit packets, SPLIT={a.split}, CK={CK}, "
-                        f"{B} N(0,1) queries per step, {NF} steps in flight (graph plans on as many streams)",
+                        f"{BPS} batches of {B} N(0,1) queries per step, {NF} batches in flight (graph plans on "
+                        f"as many streams)",
             "rows": a.rows, "cols": a.cols, "nnz": csr.nnz, "bits": a.bits, "split": a.split, "ck": CK,
-            "k": K, "queries_per_step": B, "parallelism": "single GPU",
+            "k": K, "queries_per_step": BPS * B, "batch": B, "parallelism": "single GPU",
         },
         "roofline": ({
             "bound": "hbm",
@@ -575,6 +647,7 @@ def main():
             "unit": "GB/s",
             "frac": round(bs_gbs / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_ratio": None if traffic is None else round(traffic / bs_bytes, 4),
             "algorithmic_bytes_per_launch": int(bs_bytes),
             "avg_launch_us": round(float(t_sc) * 1e3, 2),
             "mfma_int8_tops": round(bs_tops, 2),
@@ -591,10 +664,12 @@ def main():
             "avg_launch_us": round(float(t_sc) * 1e3, 2),
             "note": "int8 MACs on VALU dot4 (one useful byte lane); peak is the dense int8 matrix-core figure",
         }),
+        "roofline_hbm_slice_pass": c3,
         "roofline_single_query_slicemax": {
             "bound": "hbm", "kernel": "lowbit_slicemax", "achieved": round(s1_gbs, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(s1_gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": s1_bytes,
             "avg_launch_us": round(float(lb_ms[0]) * 1e3, 2),
+            "note": f"{s1_bytes / 1e6:.0f} MB of packets: resident in the 256 MB MALL between launches",
         },
         "cpu_baseline": cpu,
         "one_batch_at_a_time_qps": round(qps1, 2),
@@ -604,7 +679,7 @@ def main():
         "lowbit_batched_vs_fp32_speedup": round(qps1 / fqps, 3),  # both one at a time
         "batched_equals_single": same,
         "batch_launch_us": dict(zip(names, [round(float(v) * 1e3, 2) for v in bt_ms])),
-        "batch_scoring": "dense int8 tiles (MFMA)" if dense else "sliced packets (dot4)",
+        "batch_scoring": "matrix cores, slices expanded from the packets in LDS" if dense else "sliced packets (dot4)",
         "dense_bytes": g.info.dense_bytes,
         "batched_other_layout_qps": None if other_qps is None else round(other_qps, 2),
         "lowbit_launch_us": [round(float(v) * 1e3, 2) for v in lb_ms],
