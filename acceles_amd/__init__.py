@@ -48,7 +48,8 @@ ELEMENT_DTYPE = np.dtype([("value", np.float32), ("index", np.int32)])
 
 
 class PackOpts(C.Structure):
-    _fields_ = [("bits", C.c_int), ("is_signed", C.c_int), ("split", C.c_int), ("col_bits", C.c_int)]
+    _fields_ = [("bits", C.c_int), ("is_signed", C.c_int), ("split", C.c_int), ("col_bits", C.c_int),
+                ("shuffle_seed", C.c_uint64)]
 
 
 class PackedInfo(C.Structure):
@@ -56,7 +57,7 @@ class PackedInfo(C.Structure):
                 ("col_bits", C.c_int32), ("entry_bits", C.c_int32), ("is_signed", C.c_int32),
                 ("nnz", C.c_int64), ("nnz_kept", C.c_int64), ("nslices", C.c_int64),
                 ("stream_bytes", C.c_int64), ("perm_bytes", C.c_int64), ("bytes_per_nnz", C.c_double),
-                ("scale", C.c_float)]
+                ("scale", C.c_float), ("shuffle_seed", C.c_uint64)]
 
 
 class ServerOpts(C.Structure):
@@ -307,8 +308,11 @@ class CSRMatrix:
         _check(lib().acc_resparse_csr(self._h, float(keep), int(mean_over_cells), C.byref(h)))
         return CSRMatrix(h)
 
-    def pack(self, bits=8, split=32, is_signed=False, col_bits=0):
-        return PackedMatrix(self, bits=bits, split=split, is_signed=is_signed, col_bits=col_bits)
+    def pack(self, bits=8, split=32, is_signed=False, col_bits=0, shuffle_seed=0):
+        """Low-bit sliced packets; shuffle_seed != 0 deals the rows to partitions in a seeded shuffle
+        (Random-CSR), results still naming the original rows."""
+        return PackedMatrix(self, bits=bits, split=split, is_signed=is_signed, col_bits=col_bits,
+                            shuffle_seed=shuffle_seed)
 
     def __del__(self):
         if getattr(self, "_h", None) and self._h.value and _lib is not None:
@@ -466,8 +470,8 @@ def xdecode(flags, n=30, carry=0):
 class PackedMatrix:
     """Quantized, sliced low-bit packets of a CSR matrix (acc_pack_lowbit)."""
 
-    def __init__(self, csr: CSRMatrix, bits=8, split=32, is_signed=False, col_bits=0):
-        o = PackOpts(bits, int(is_signed), split, col_bits)
+    def __init__(self, csr: CSRMatrix, bits=8, split=32, is_signed=False, col_bits=0, shuffle_seed=0):
+        o = PackOpts(bits, int(is_signed), split, col_bits, int(shuffle_seed))
         h = _P()
         _check(lib().acc_pack_lowbit(csr._h, C.byref(o), C.byref(h)))
         self._h = h

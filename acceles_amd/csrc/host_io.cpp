@@ -558,12 +558,19 @@ extern "C" int acc_generate_queries(int32_t nq, int32_t cols, int val_kind, int 
 // Random-CSR: seeded row permutation
 // ---------------------------------------------------------------------------
 
-extern "C" int acc_permute_rows(const acc_csr *m, uint64_t seed, acc_csr **out, int32_t *perm) {
-    if (!m || !out) return fail(ACC_ERR_ARG, "null argument");
-    std::vector<int32_t> p((size_t)m->rows);
+namespace acc {
+std::vector<int32_t> row_shuffle(int32_t rows, uint64_t seed) {
+    std::vector<int32_t> p((size_t)rows);
     std::iota(p.begin(), p.end(), 0);
     Rng g(seed, 0xFFFFFFFFull);
-    for (int32_t i = m->rows - 1; i > 0; --i) std::swap(p[(size_t)i], p[(size_t)g.below((uint64_t)i + 1)]);
+    for (int32_t i = rows - 1; i > 0; --i) std::swap(p[(size_t)i], p[(size_t)g.below((uint64_t)i + 1)]);
+    return p;
+}
+}  // namespace acc
+
+extern "C" int acc_permute_rows(const acc_csr *m, uint64_t seed, acc_csr **out, int32_t *perm) {
+    if (!m || !out) return fail(ACC_ERR_ARG, "null argument");
+    const std::vector<int32_t> p = acc::row_shuffle(m->rows, seed);
     acc_csr *o = new acc_csr();
     o->rows = m->rows;
     o->cols = m->cols;

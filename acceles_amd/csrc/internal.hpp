@@ -33,6 +33,7 @@ struct acc_packed {
     float scale = 0.f;
     int64_t nnz = 0, nnz_kept = 0;
     int64_t dup_rows = 0;                   // rows holding a column twice among kept entries
+    uint64_t shuffle_seed = 0;              // Random-CSR: rows dealt to partitions in row_shuffle order (0: none)
     int32_t merged_max = 0, merged_min = 0; // range of the per (row, column) sums of kept values
     std::vector<int32_t> part_row_begin;    // split + 1
     std::vector<int64_t> part_slice_begin;  // split + 1
@@ -60,6 +61,9 @@ inline uint32_t ordered_f32(float f) {
 }
 
 int bits_for(int32_t n);  // ceil(log2(n)), at least 1
+// Random-CSR row order: the seeded Fisher-Yates shuffle of 0 .. rows-1 that
+// acc_permute_rows applies (out row i = in row order[i]).
+std::vector<int32_t> row_shuffle(int32_t rows, uint64_t seed);
 
 // Device-memory admission: byte reservations against a fixed limit, each
 // checked and taken under one lock, so concurrent callers cannot both pass

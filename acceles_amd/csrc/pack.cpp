@@ -126,18 +126,30 @@ extern "C" int acc_pack_lowbit(const acc_csr *m, const acc_pack_opts *o, acc_pac
     pk->slice_ndw.assign((size_t)NS, 0);
     pk->slice_off.assign((size_t)NS, 0);
 
-    // order rows of each partition by kept count descending, row ascending
-    // (counting sort: stable, O(rows))
+    // Random-CSR (README.md:5; quantify_util.py:491 shuffles the rows before
+    // the partitions): with a seed, partition p holds rows order[b .. e) of
+    // the seeded shuffle instead of rows b .. e-1.  Rows keep their ids (the
+    // slot -> row map names them), so results and ties are unchanged in kind.
+    pk->shuffle_seed = o->shuffle_seed;
+    std::vector<int32_t> order;
+    if (o->shuffle_seed) order = acc::row_shuffle(R, o->shuffle_seed);
+    const int32_t *ord = order.empty() ? nullptr : order.data();
+    // order rows of each partition by kept count descending, then by their
+    // place in the partition (counting sort: stable, O(rows))
 #pragma omp parallel for schedule(dynamic)
     for (int p = 0; p < S; ++p) {
         int32_t b = pk->part_row_begin[(size_t)p], e = pk->part_row_begin[(size_t)p + 1];
+        auto row_at = [&](int32_t i) { return ord ? ord[i] : i; };
         int32_t maxc = 0;
-        for (int32_t r = b; r < e; ++r) maxc = std::max(maxc, kept[(size_t)r]);
+        for (int32_t i = b; i < e; ++i) maxc = std::max(maxc, kept[(size_t)row_at(i)]);
         std::vector<int64_t> start((size_t)maxc + 2, 0);
-        for (int32_t r = b; r < e; ++r) start[(size_t)(maxc - kept[(size_t)r]) + 1]++;
+        for (int32_t i = b; i < e; ++i) start[(size_t)(maxc - kept[(size_t)row_at(i)]) + 1]++;
         for (int32_t c = 0; c <= maxc; ++c) start[(size_t)c + 1] += start[(size_t)c];
         int32_t *dst = pk->perm.data() + pk->part_slice_begin[(size_t)p] * 64;
-        for (int32_t r = b; r < e; ++r) dst[start[(size_t)(maxc - kept[(size_t)r])]++] = r;
+        for (int32_t i = b; i < e; ++i) {
+            const int32_t r = row_at(i);
+            dst[start[(size_t)(maxc - kept[(size_t)r])]++] = r;
+        }
         for (int64_t s = pk->part_slice_begin[(size_t)p]; s < pk->part_slice_begin[(size_t)p + 1]; ++s) {
             int32_t r0 = pk->perm[(size_t)s * 64];  // longest row of the slice
             int64_t cnt = r0 >= 0 ? kept[(size_t)r0] : 0;
@@ -194,6 +206,7 @@ extern "C" int acc_packed_info(const acc_packed *pk, acc_packed_info_t *info) {
     info->perm_bytes = (int64_t)pk->perm.size() * 4;
     info->bytes_per_nnz = pk->nnz > 0 ? (double)info->stream_bytes / (double)pk->nnz : 0.0;
     info->scale = pk->scale;
+    info->shuffle_seed = pk->shuffle_seed;
     return ACC_OK;
 }
 
@@ -331,7 +344,7 @@ extern "C" int acc_unpack_ultracsr(const uint64_t *packets, int64_t nnz, int col
 namespace {
 
 const char kCsrMagic[8] = {'A', 'C', 'C', 'C', 'S', 'R', '0', '1'};
-const char kPackMagic[8] = {'A', 'C', 'C', 'P', 'K', 'T', '0', '1'};
+const char kPackMagic[8] = {'A', 'C', 'C', 'P', 'K', 'T', '0', '2'};
 
 struct File {
     FILE *f = nullptr;
@@ -519,7 +532,7 @@ extern "C" int acc_packed_save(const acc_packed *pk, const char *path) {
               put(w.f, pk->split) && put(w.f, pk->bits) && put(w.f, pk->col_bits) && put(w.f, pk->entry_bits) &&
               put(w.f, pk->is_signed) && put(w.f, pk->scale) && put(w.f, pk->nnz) && put(w.f, pk->nnz_kept) &&
               put(w.f, pk->dup_rows) && put(w.f, pk->merged_max) && put(w.f, pk->merged_min) &&
-              put_vec(w.f, pk->part_row_begin) && put_vec(w.f, pk->part_slice_begin) && put_vec(w.f, pk->slice_off) &&
+              put(w.f, pk->shuffle_seed) && put_vec(w.f, pk->part_row_begin) && put_vec(w.f, pk->part_slice_begin) && put_vec(w.f, pk->slice_off) &&
               put_vec(w.f, pk->slice_ndw) && put_vec(w.f, pk->perm) && put_vec(w.f, pk->words);
     return ok ? ACC_OK : fail(ACC_ERR_IO, std::string("write failed: ") + path);
 }
@@ -535,7 +548,7 @@ extern "C" int acc_packed_load(const char *path, acc_packed **out) {
                   get(r.f, pk->cols) && get(r.f, pk->split) && get(r.f, pk->bits) && get(r.f, pk->col_bits) &&
                   get(r.f, pk->entry_bits) && get(r.f, pk->is_signed) && get(r.f, pk->scale) && get(r.f, pk->nnz) &&
                   get(r.f, pk->nnz_kept) && get(r.f, pk->dup_rows) && get(r.f, pk->merged_max) &&
-                  get(r.f, pk->merged_min) && pk->split >= 1 && pk->rows >= pk->split && pk->col_bits >= 1 &&
+                  get(r.f, pk->merged_min) && get(r.f, pk->shuffle_seed) && pk->split >= 1 && pk->rows >= pk->split && pk->col_bits >= 1 &&
                   pk->col_bits <= 20 && get_vec(r.f, pk->part_row_begin, (uint64_t)pk->split + 1) &&
                   get_vec(r.f, pk->part_slice_begin, (uint64_t)pk->split + 1) &&
                   get_vec(r.f, pk->slice_off, UINT64_MAX / 8) && get_vec(r.f, pk->slice_ndw, pk->slice_off.size()) &&

@@ -170,6 +170,11 @@ typedef struct {
                       1: symmetric signed [-2^(B-1), 2^(B-1)-1]              */
     int split;     /* SPLIT row partitions (quantify_util.py:184-189)        */
     int col_bits;  /* 0 = ceil(log2(cols))                                   */
+    uint64_t shuffle_seed; /* Random-CSR (README.md:5; quantify_util.py:491
+                      shuffles rows before the partitions of :184-189): 0 =
+                      partitions of consecutive rows; else partition p holds
+                      rows order[p*size ...] of acc_permute_rows(seed)'s
+                      order.  Results name the original rows either way.    */
 } acc_pack_opts;
 
 typedef struct {
@@ -181,6 +186,7 @@ typedef struct {
     int64_t perm_bytes;   /* slot -> row map (read only for candidates)      */
     double bytes_per_nnz; /* stream_bytes / nnz                               */
     float scale;          /* global FP32 scale of the values                 */
+    uint64_t shuffle_seed; /* acc_pack_opts.shuffle_seed (0: no shuffle)     */
 } acc_packed_info_t;
 
 /* Quantize (acc_quantize rules) and pack the CSR (host_spmv_ucsr.cpp:278-319

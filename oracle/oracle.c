@@ -225,6 +225,11 @@ int orc_spmv_i32(int32_t nrows, const int64_t *ptr, const int32_t *idx,
 
 int orc_split_select(int32_t nrows, const int32_t *y, int split, int ck,
                      int32_t *cand_rows, int32_t *cand_scores) {
+    return orc_split_select_order(nrows, y, NULL, split, ck, cand_rows, cand_scores);
+}
+
+int orc_split_select_order(int32_t nrows, const int32_t *y, const int32_t *order, int split, int ck,
+                           int32_t *cand_rows, int32_t *cand_scores) {
     if (split < 1 || ck < split || nrows < split) return -1;
     int32_t size = nrows / split;
     int m = ck / split;
@@ -234,7 +239,11 @@ int orc_split_select(int32_t nrows, const int32_t *y, int split, int ck,
     for (int p = 0; p < split; ++p) {
         int32_t b = p * size;
         int32_t e = (p == split - 1) ? nrows : b + size;
-        for (int32_t r = b; r < e; ++r) { items[r - b].s = y[r]; items[r - b].r = r; }
+        for (int32_t i = b; i < e; ++i) {
+            const int32_t r = order ? order[i] : i;
+            items[i - b].s = y[r];
+            items[i - b].r = r;
+        }
         int got = heap_topk(items, e - b, m, out);
         for (int i = 0; i < got; ++i) {
             cand_rows[n] = out[i].r;
@@ -271,6 +280,13 @@ int orc_two_stage(int32_t nrows, int32_t ncols, const int64_t *ptr,
                   const int32_t *idx, const float *val, const int32_t *qv,
                   const float *x, int G, int split, int ck, int k,
                   int32_t *cand_rows, int32_t *out_idx, float *out_val) {
+    return orc_two_stage_order(nrows, ncols, ptr, idx, val, qv, x, G, split, ck, k, NULL, cand_rows, out_idx,
+                               out_val);
+}
+
+int orc_two_stage_order(int32_t nrows, int32_t ncols, const int64_t *ptr, const int32_t *idx, const float *val,
+                        const int32_t *qv, const float *x, int G, int split, int ck, int k, const int32_t *order,
+                        int32_t *cand_rows, int32_t *out_idx, float *out_val) {
     int32_t *qx = (int32_t *)malloc(sizeof(int32_t) * (size_t)ncols);
     float sx;
     orc_quantize(ncols, x, 8, 1, &sx, qx);
@@ -279,7 +295,7 @@ int orc_two_stage(int32_t nrows, int32_t ncols, const int64_t *ptr,
     int n = -1;
     if (rc == 0) {
         int32_t *cs = (int32_t *)malloc(sizeof(int32_t) * (size_t)ck);
-        n = orc_split_select(nrows, y, split, ck, cand_rows, cs);
+        n = orc_split_select_order(nrows, y, order, split, ck, cand_rows, cs);
         if (n >= 0) orc_rescore_topk(ptr, idx, val, x, G, n, cand_rows, k, out_idx, out_val);
         free(cs);
     }

@@ -32,6 +32,9 @@
  *                          golden fixtures
  *   orc_split_select       per-partition brute force, partition sizes of
  *                          quantify_util.py:185-189
+ *   orc_split_select_order numpy lexsort brute force per partition of a
+ *                          shuffled order (ties among equal scores included);
+ *                          identity order equals orc_split_select
  *   orc_rescore_topk       orc_spmv_f32_grouped + orc_topk on the candidate rows
  *   orc_precision          hand-computed examples of quantify_util.py:198-241
  *   orc_ndcg_score         hand-computed lists (equal lists give 1, a
@@ -92,6 +95,13 @@ void orc_topk(int64_t n, const double *y, int k, int32_t *out_idx, double *out_v
 int orc_split_select(int32_t nrows, const int32_t *y, int split, int ck,
                      int32_t *cand_rows, int32_t *cand_scores);
 
+/* Random-CSR (README.md:5; quantify_util.py:491 shuffles the rows before the
+ * partitions of :184-189): the same select with the rows dealt to partitions
+ * in `order` (partition p holds rows order[p*size ...]; NULL = 0, 1, ...).
+ * Rows keep their ids: ties still go to the lower row id. */
+int orc_split_select_order(int32_t nrows, const int32_t *y, const int32_t *order, int split, int ck,
+                           int32_t *cand_rows, int32_t *cand_scores);
+
 /* Stage 2 (quantify_util.py:354-367, in FP32): rescore the candidate rows
  * against the unquantized query with orc_spmv_f32_grouped's order, then
  * keep the top k in ranking order. */
@@ -108,6 +118,10 @@ int orc_two_stage(int32_t nrows, int32_t ncols, const int64_t *ptr,
                   const int32_t *idx, const float *val, const int32_t *qv,
                   const float *x, int G, int split, int ck, int k,
                   int32_t *cand_rows, int32_t *out_idx, float *out_val);
+/* orc_two_stage over the partitions of a row order (orc_split_select_order). */
+int orc_two_stage_order(int32_t nrows, int32_t ncols, const int64_t *ptr, const int32_t *idx, const float *val,
+                        const int32_t *qv, const float *x, int G, int split, int ck, int k, const int32_t *order,
+                        int32_t *cand_rows, int32_t *out_idx, float *out_val);
 
 /* Precision of one query (quantify_util.py:198-219): for each K in ks,
  * |set(cand) & set(exact[0:K])| / K. */

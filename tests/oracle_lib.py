@@ -47,6 +47,9 @@ def oracle() -> C.CDLL:
         L.orc_topk.argtypes = [C.c_int64, _F64, C.c_int, _I32, _F64]
         L.orc_topk.restype = None
         L.orc_split_select.argtypes = [C.c_int32, _I32, C.c_int, C.c_int, _I32, _I32]
+        L.orc_split_select_order.argtypes = [C.c_int32, _I32, _I32, C.c_int, C.c_int, _I32, _I32]
+        L.orc_two_stage_order.argtypes = [C.c_int32, C.c_int32, _I64, _I32, _F32, _I32, _F32, C.c_int, C.c_int,
+                                          C.c_int, C.c_int, _I32, _I32, _I32, _F32]
         L.orc_rescore_topk.argtypes = [_I64, _I32, _F32, _F32, C.c_int, C.c_int, _I32, C.c_int, _I32, _F32]
         L.orc_rescore_topk.restype = None
         L.orc_two_stage.argtypes = [C.c_int32, C.c_int32, _I64, _I32, _F32, _I32, _F32, C.c_int, C.c_int, C.c_int,
@@ -174,6 +177,18 @@ def split_select(y, split, ck):
     return rows[:n], sc[:n]
 
 
+def split_select_order(y, order, split, ck):
+    """split_select with the rows dealt to partitions in `order` (Random-CSR)."""
+    y = np.ascontiguousarray(y, np.int32)
+    order = np.ascontiguousarray(order, np.int32)
+    rows = np.empty(ck, np.int32)
+    sc = np.empty(ck, np.int32)
+    n = oracle().orc_split_select_order(len(y), y, order, split, ck, rows, sc)
+    if n < 0:
+        raise ValueError("bad split/ck")
+    return rows[:n], sc[:n]
+
+
 def rescore_topk(ptr, idx, val, x, G, cand, k):
     oi = np.empty(k, np.int32)
     ov = np.empty(k, np.float32)
@@ -246,6 +261,21 @@ def top_k_spmv_f32_par(ptr, idx, val, x, k, nthreads):
     ov = np.empty(k, np.float32)
     oracle().orc_top_k_spmv_f32_par(n, ptr, idx, val, x, k, nthreads, y, oi, ov)
     return oi, ov
+
+
+def two_stage_order(ptr, idx, val, qv, x, cols, G, split, ck, k, order):
+    """two_stage over the partitions of a row order (Random-CSR)."""
+    n = len(ptr) - 1
+    cand = np.empty(ck, np.int32)
+    oi = np.empty(k, np.int32)
+    ov = np.empty(k, np.float32)
+    m = oracle().orc_two_stage_order(n, cols, np.ascontiguousarray(ptr, np.int64), np.ascontiguousarray(idx, np.int32),
+                                     np.ascontiguousarray(val, np.float32), np.ascontiguousarray(qv, np.int32),
+                                     np.ascontiguousarray(x, np.float32), G, split, ck, k,
+                                     np.ascontiguousarray(order, np.int32), cand, oi, ov)
+    if m < 0:
+        raise ValueError("two-stage failed")
+    return cand[:m], oi, ov
 
 
 def two_stage_par(ptr, idx, val, qv, x, cols, G, split, ck, k, nthreads):

@@ -59,11 +59,11 @@ def test_malformed_files_refused(acc, tmp_path):
     corrupt(bad, off, np.int32(50).tobytes())
     with pytest.raises(acc.AccelesError, match="not a CSR file"):
         acc.CSRMatrix.load(bad)
-    # a lane naming a row past the matrix: perm[0], after the 72-byte header and four vectors
+    # a lane naming a row past the matrix: perm[0], after the 80-byte header and four vectors
     badk = str(tmp_path / "bad.pk")
     pk.save(badk)
     split, ns = 4, pk.info.nslices
-    off = 72 + (8 + 4 * (split + 1)) + (8 + 8 * (split + 1)) + (8 + 4 * ns) + (8 + 4 * ns) + 8
+    off = 80 + (8 + 4 * (split + 1)) + (8 + 8 * (split + 1)) + (8 + 4 * ns) + (8 + 4 * ns) + 8
     corrupt(badk, off, np.int32(200).tobytes())
     with pytest.raises(acc.AccelesError, match="not a packet file"):
         acc.PackedMatrix.load(badk)
@@ -74,9 +74,10 @@ def test_malformed_files_refused(acc, tmp_path):
 
 
 def packet_offsets(pk, split):
-    """Byte offsets of the vectors of a packet file (each: a uint64 length, then the data)."""
+    """Byte offsets of the vectors of a packet file (each: a uint64 length, then the data), after the
+    80-byte header (magic, 7 int32 fields, scale, nnz, nnz_kept, dup_rows, merged max/min, shuffle seed)."""
     ns = pk.info.nslices
-    o = {"part_row_begin": 72}
+    o = {"part_row_begin": 80}
     o["part_slice_begin"] = o["part_row_begin"] + 8 + 4 * (split + 1)
     o["slice_off"] = o["part_slice_begin"] + 8 + 8 * (split + 1)
     o["slice_ndw"] = o["slice_off"] + 8 + 4 * ns

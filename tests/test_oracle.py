@@ -245,6 +245,28 @@ def test_split_select_equals_per_partition_bruteforce(nrows, split, ck):
     np.testing.assert_array_equal(scores, y[rows])
 
 
+@pytest.mark.parametrize("nrows,split,ck", [(1000, 8, 64), (777, 5, 40), (300, 3, 300)])
+def test_split_select_order_equals_bruteforce_over_the_shuffled_partitions(nrows, split, ck):
+    """Random-CSR select: partition p holds rows order[p*size ...]; within a partition the rows rank by
+    score descending, then original row ascending (ties are frequent: 100 score values)."""
+    rng = np.random.default_rng(nrows * split)
+    y = rng.integers(-50, 50, nrows).astype(np.int32)
+    order = rng.permutation(nrows).astype(np.int32)
+    rows, scores = O.split_select_order(y, order, split, ck)
+    size, m = nrows // split, ck // split
+    expect = []
+    for p in range(split):
+        part = order[p * size:] if p == split - 1 else order[p * size:(p + 1) * size]
+        expect += part[np.lexsort((part, -y[part]))][:m].tolist()
+    np.testing.assert_array_equal(rows, expect)
+    np.testing.assert_array_equal(scores, y[rows])
+    # the identity order is the plain select
+    r0, s0 = O.split_select_order(y, np.arange(nrows, dtype=np.int32), split, ck)
+    r1, s1 = O.split_select(y, split, ck)
+    np.testing.assert_array_equal(r0, r1)
+    np.testing.assert_array_equal(s0, s1)
+
+
 def test_split_select_rejects_bad_arguments():
     with pytest.raises(ValueError):
         O.split_select(np.zeros(10, np.int32), 32, 512)
