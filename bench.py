@@ -58,13 +58,15 @@ def parse():
                          "0: 3 batches (embedding), 2 batches (batched), 2 queries (large-single)")
     ap.add_argument("--batch-mode", choices=["auto", "packets", "dense"], default="auto",
                     help="scoring layout of the batched path")
-    ap.add_argument("--workload", choices=["embedding", "large-single", "batched", "serve", "copy", "resparse"],
+    ap.add_argument("--workload", choices=["embedding", "large-single", "batched", "serve", "copy", "resparse",
+                                           "bitwidth"],
                     default="embedding",
                     help="embedding: 400,000x500 batched (default line); large-single: 20,000,000x1024 single "
                          "queries; batched: 2,000,000x900, 256 queries of mixed K per graph launch; serve: three "
                          "resident matrices under Poisson arrivals; copy: 1 GiB device-to-device copies, the "
                          "calibration run for the HBM traffic counters; resparse: the ReSparse keep-ratio search on "
-                         "the embedding matrix, then single queries on the pruned packets")
+                         "the embedding matrix, then single queries on the pruned packets; bitwidth: "
+                         "precision/recall@K against bytes/nnz for 4-, 6- and 8-bit packets")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="serve: seconds per offered rate")
     ap.add_argument("--deg-dist", choices=["uniform", "gamma"], default="uniform",
                     help="large-single: row lengths U[10,30] or Gamma(0.5, 40) (mean 20)")
@@ -378,6 +380,71 @@ def bench_serve(a, acc, stream):
     print(json.dumps(line), flush=True)
 
 
+def topk_rows(y, k):
+    """Top-k rows of a score vector: score descending, row ascending."""
+    top = np.argpartition(-y, k)[:k + 1] if len(y) > k else np.arange(len(y))
+    top = top[np.lexsort((top, -y[top]))]
+    return top[:k]
+
+
+def bench_bitwidth(a, acc, stream):
+    """Precision/recall@K against bytes/nnz per value width (4-bit unsigned, 6-bit signed -- the
+    reference's B of types.hpp:148 -- and 8-bit unsigned) on the embedding matrix (400,000x500,
+    SPLIT=32, CK=512) and the 20,000,000x1024 matrix (SPLIT=64, CK=1024).  Per width: the stage-1
+    approximate Top-K from the INT32 scores alone (the quantity of plot_quantize_with_4to6bit.py:14-16)
+    and the two-stage result after FP32 rescoring, both against the FP64 exact Top-K."""
+    configs = [("400000x500", 400000, 500, (40, 100), True, 32, 512, 64),
+               ("20000000x1024", 20_000_000, 1024, (10, 30), False, 64, 1024, 16)]
+    widths = [(4, False), (6, True), (8, False)]
+    K = 100
+    table = []
+    for name, rows, cols, deg, l2, split, ck, nq in configs:
+        t0 = time.perf_counter()
+        csr = acc.CSRMatrix.generate(rows, cols, deg=deg, l2norm=l2, seed=0xACCE1E5 + (1 if l2 else 2))
+        X = acc.generate_queries(nq, cols, seed=0xACCE1E5 + 9)
+        Xd = acc.DeviceBuffer.from_numpy(X)
+        xs = [Xd.at(i * cols) for i in range(nq)]
+        exact = None
+        for bits, signed in widths:
+            pk = csr.pack(bits=bits, split=split, is_signed=signed)
+            g = acc.GpuMatrix(csr, pk, device=0)
+            if exact is None:
+                exact = [exact_topk(g, acc, xs[i], rows, K, stream) for i in range(nq)]
+            y = acc.DeviceBuffer((rows,), np.int32)
+            out = acc.DeviceBuffer((K, 2), np.int32)
+            p1, p2 = [], []
+            for i in range(nq):
+                g.scores_lowbit(xs[i], y, stream)
+                stream.synchronize()
+                p1.append(precision_at(topk_rows(y.numpy().astype(np.int64), K), exact[i]))
+                g.topk_lowbit(xs[i], K, ck, out, stream)
+                stream.synchronize()
+                p2.append(precision_at(out.numpy()[:, 1], exact[i]))
+            ms = g.time_lowbit_stage(xs[0], K, ck, 0, 20, stream)
+            row = {"matrix": name, "bits": bits, "signed": signed, "bytes_per_nnz": round(pk.info.bytes_per_nnz, 4),
+                   "packed_bytes": int(pk.info.stream_bytes), "slicemax_us": round(ms * 1e3, 2),
+                   "stage1_precision_at_k": dict(zip(map(str, KS), np.round(np.mean(p1, axis=0), 4).tolist())),
+                   "two_stage_precision_at_k": dict(zip(map(str, KS), np.round(np.mean(p2, axis=0), 4).tolist())),
+                   "two_stage_all_found_at_k": dict(zip(map(str, KS), np.round(np.mean(np.array(p2) == 1.0, axis=0),
+                                                                               4).tolist())),
+                   "queries": nq}
+            table.append(row)
+            log(f"[bench] {name} {bits}-bit{' signed' if signed else ''}: {row['bytes_per_nnz']} B/nnz, slicemax "
+                f"{row['slicemax_us']}us; stage-1 P@K {row['stage1_precision_at_k']}; two-stage P@K "
+                f"{row['two_stage_precision_at_k']}")
+            del g, pk, y
+        log(f"[bench] {name}: {time.perf_counter() - t0:.1f}s")
+        del csr
+    c2_4 = next(r for r in table if r["matrix"] == "400000x500" and r["bits"] == 4)
+    line = line_common(a, "two-stage precision@100 at 4-bit (embedding matrix)", c2_4["two_stage_precision_at_k"]["100"],
+                       "precision", 0.0, "u4",
+                       "precision/recall@K vs bytes/nnz for 4-, 6- (signed) and 8-bit packets on 400000x500 "
+                       "(SPLIT=32, CK=512) and 20000000x1024 (SPLIT=64, CK=1024); K=1..100 vs FP64 exact",
+                       {})
+    line.update({"roofline": None, "cpu_baseline": None, "table": table})
+    print(json.dumps(line), flush=True)
+
+
 def bench_copy(a, acc, stream):
     """1 GiB device-to-device copies: rocprofv3 FETCH_SIZE / WRITE_SIZE of this copy kernel against its known
     size (1 GiB read, 1 GiB written per copy) give the counters' scale on this GPU."""
@@ -459,7 +526,7 @@ def main():
     if a.workload != "embedding":
         stream = acc.Stream(0)
         {"large-single": bench_large_single, "batched": bench_batched, "serve": bench_serve,
-         "copy": bench_copy, "resparse": bench_resparse}[a.workload](a, acc, stream)
+         "copy": bench_copy, "resparse": bench_resparse, "bitwidth": bench_bitwidth}[a.workload](a, acc, stream)
         return
 
     assert acc.device_count() > 0, "no GPU visible"
