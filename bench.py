@@ -281,9 +281,16 @@ def bench_batched(a, acc, stream):
     rows, cols, B, CK = 2_000_000, 900, 256, 512
     t0 = time.perf_counter()
     csr = acc.CSRMatrix.generate(rows, cols, deg=(30, 90), seed=0xACCE1E5 + 4)
-    pk = csr.pack(bits=a.bits, split=a.split)
+    # partition count (SPLIT) and scoring layout tuned under recall@100 >= 0.99 (acceles_amd/autotune.py)
+    from acceles_amd.autotune import tune_lowbit
+    best, tpts = tune_lowbit(csr, bits=a.bits, k=100, splits=(16, 32, 64), cks=(CK,), batches=(B,), nq=64,
+                             floor=0.99, reps=3, log=log)
+    split = best.split if best is not None else a.split
+    a.split = split
+    pk = csr.pack(bits=a.bits, split=split)
     g = acc.GpuMatrix(csr, pk, device=0)
-    log(f"[bench] setup {time.perf_counter() - t0:.1f}s: nnz={csr.nnz} packed={pk.info.stream_bytes / 1e6:.0f}MB")
+    log(f"[bench] setup {time.perf_counter() - t0:.1f}s: nnz={csr.nnz} packed={pk.info.stream_bytes / 1e6:.0f}MB, "
+        f"tuned SPLIT={split}")
     pts = g.autotune(CK, 100, B)
     log("[bench] autotune: " + ", ".join(f"nq={p.nq}:{['auto', 'packets', 'dense', 'single'][p.layout]}"
                                          f"({p.us_single:.0f}/{p.us_sparse:.0f}/{p.us_dense:.0f}us)" for p in pts))
@@ -326,6 +333,7 @@ def bench_batched(a, acc, stream):
                        f"{NF} launches in flight (a plan per stream)",
                        {"rows": rows, "cols": cols, "nnz": csr.nnz, "split": a.split, "queries_per_step": B})
     line.update({"roofline": None, "cpu_baseline": None, "mean_precision_at_k": round(float(np.mean(prec)), 5),
+                 "tuned": [p.as_dict() for p in tpts],
                  "one_at_a_time_qps": round(a.steps * B / dt1, 2), "in_flight_equals_one_at_a_time": same,
                  "autotune": [{"nq": p.nq, "layout": int(p.layout), "us_single": round(p.us_single, 1),
                                "us_packets": round(p.us_sparse, 1), "us_dense": round(p.us_dense, 1)} for p in pts]})
