@@ -62,13 +62,14 @@ class PackedInfo(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("memory_budget", C.c_int64), ("max_batch", C.c_int), ("slots", C.c_int),
-                ("ck", C.c_int), ("max_pending", C.c_int), ("autotune", C.c_int)]
+                ("ck", C.c_int), ("max_pending", C.c_int), ("autotune", C.c_int), ("max_request_bytes", C.c_int64)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("refused", C.c_int64), ("batches", C.c_int64),
                 ("mean_batch", C.c_double), ("p50_us", C.c_double), ("p99_us", C.c_double), ("max_us", C.c_double),
-                ("qps", C.c_double), ("device_bytes", C.c_int64)]
+                ("qps", C.c_double), ("device_bytes", C.c_int64), ("request_bytes", C.c_int64),
+                ("peak_inflight", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -650,8 +651,9 @@ class Plan:
 class Server:
     """Request loop over resident matrices with continuous batching (acc_server_*)."""
 
-    def __init__(self, device=0, memory_budget=0, max_batch=256, slots=4, ck=512, max_pending=0, autotune=False):
-        o = ServerOpts(device, memory_budget, max_batch, slots, ck, max_pending, int(autotune))
+    def __init__(self, device=0, memory_budget=0, max_batch=256, slots=4, ck=512, max_pending=0, autotune=False,
+                 max_request_bytes=0):
+        o = ServerOpts(device, memory_budget, max_batch, slots, ck, max_pending, int(autotune), int(max_request_bytes))
         h = _P()
         _check(lib().acc_server_create(C.byref(o), C.byref(h)))
         self._h = h

@@ -913,6 +913,42 @@ static int check_batch(acc_gpu_matrix *m, const float *x_dev, int nq, const int3
     return check_lowbit(m, ck, *kmax);
 }
 
+// A batch workspace of its own, for callers that keep several batches of one
+// matrix in flight (the server's slots); like a plan's, grown on demand.
+namespace acc {
+struct BatchWorkspace {
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    BatchWs ws;
+};
+
+BatchWorkspace *batch_ws_create() {
+    BatchWorkspace *w = new BatchWorkspace();
+    w->ws.allocs = &w->allocs;
+    w->ws.bytes = &w->bytes;
+    return w;
+}
+
+void batch_ws_destroy(BatchWorkspace *w, int device) {
+    if (!w) return;
+    hipSetDevice(device);
+    for (void *q : w->allocs) hipFree(q);
+    delete w;
+}
+
+int64_t batch_ws_bytes(const BatchWorkspace *w) { return w ? w->bytes : 0; }
+
+int topk_lowbit_batch_ws(acc_gpu_matrix *m, BatchWorkspace *w, const float *x_dev, int nq, const int32_t *k_host,
+                         int ck, acc_element *out_dev, int out_stride, void *stream) {
+    if (!w) return fail(ACC_ERR_ARG, "null workspace");
+    int kmax = 1;
+    int rc = check_batch(m, x_dev, nq, k_host, ck, out_dev, out_stride, &kmax);
+    if (rc != ACC_OK || nq == 0) return rc;
+    HIP_TRY(hipSetDevice(m->device));
+    return run_batch(m, w->ws, x_dev, nq, k_host, true, kmax, ck, out_dev, out_stride, (hipStream_t)stream);
+}
+}  // namespace acc
+
 extern "C" int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq, const int32_t *k_host, int ck,
                                          acc_element *out_dev, int out_stride, void *stream) {
     int kmax = 1;

@@ -61,6 +61,17 @@ inline uint32_t ordered_f32(float f) {
 }
 
 int bits_for(int32_t n);  // ceil(log2(n)), at least 1
+// Batched low-bit queries with a workspace of the caller's (acc_gpu_topk_
+// lowbit_batch uses the handle's own, so one handle runs one batch at a
+// time): several workspaces keep several batches of one matrix in flight on
+// different streams.  The workspace grows on demand; bytes reports it.
+struct BatchWorkspace;
+BatchWorkspace *batch_ws_create();
+void batch_ws_destroy(BatchWorkspace *w, int device);
+int64_t batch_ws_bytes(const BatchWorkspace *w);
+int topk_lowbit_batch_ws(acc_gpu_matrix *m, BatchWorkspace *w, const float *x_dev, int nq, const int32_t *k_host,
+                         int ck, acc_element *out_dev, int out_stride, void *stream);
+
 // Random-CSR row order: the seeded Fisher-Yates shuffle of 0 .. rows-1 that
 // acc_permute_rows applies (out row i = in row order[i]).
 std::vector<int32_t> row_shuffle(int32_t rows, uint64_t seed);

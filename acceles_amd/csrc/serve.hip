@@ -5,15 +5,17 @@
 // read back (host_spmv_ucsr.cpp:1084-1180; cpu_mkl_baseline.cpp:254-272).
 // Here a worker thread keeps every registered matrix resident in device
 // memory and turns the queue of arriving requests into batches:
-//   * a matrix has at most one batch in flight (its handle's workspace and
-//     selection state are reused between batches); batches of different
-//     matrices run concurrently on separate streams ("slots");
+//   * batches run concurrently on separate streams ("slots"), each slot with
+//     a batch workspace of its own, so a matrix may have as many batches in
+//     flight as there are slots;
 //   * whenever a slot is free, the matrix whose oldest waiting request is the
 //     oldest gets the slot and up to max_batch of its waiting requests, each
-//     with its own K -- requests that arrive while a batch runs are admitted
-//     into the next one as soon as the slot frees (continuous batching);
-//   * matrices are admitted against a device-memory budget that includes the
-//     workspace of a full batch; requests against the queue capacity.
+//     with its own K -- requests that arrive while batches run are admitted
+//     into the next one as soon as a slot frees (continuous batching);
+//   * matrices are admitted against a device-memory budget that includes
+//     every slot's workspace for a full batch of the matrix; requests against
+//     a host-memory budget (the query, its results and bookkeeping) and the
+//     queue capacity.
 // Results are those of acc_gpu_topk_lowbit_batch, which do not depend on the
 // batch a query lands in.
 #include "internal.hpp"
@@ -41,6 +43,7 @@ constexpr int kMaxSlots = 8;
 constexpr int kMaxK = 960;
 
 struct Request {
+    int64_t bytes = 0;  // charged to the request-memory budget
     int matrix = 0;
     int k = 0;
     bool keep = true;  // false: results are discarded (load generator)
@@ -55,11 +58,12 @@ struct Matrix {
     acc_gpu_matrix *m = nullptr;
     int cols = 0;
     std::deque<uint64_t> pending;
-    bool inflight = false;
+    int inflight = 0;   // batches on the slots
     int64_t bytes = 0;
 };
 
 struct Slot {
+    acc::BatchWorkspace *ws = nullptr;  // this slot's batch workspace (any matrix)
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
     float *x_pin = nullptr;        // max_batch x max_cols
@@ -88,15 +92,24 @@ struct acc_server {
     acc::Budget mem;  // device bytes: slot buffers and resident matrices with their workspaces
     int max_cols = 8192;
     int64_t npending = 0;
+    int64_t ws_need = 0;    // the largest full-batch workspace of a resident matrix (reserved per slot)
+    int64_t req_bytes = 0;  // host bytes held by waiting and unread requests
     // statistics since the last reset
     std::vector<double> lat_us;
     int64_t submitted = 0, completed = 0, refused = 0, batches = 0, batched = 0;
+    int64_t peak_inflight = 0;  // most batches of one matrix on the slots at once
     Clock::time_point t_reset = Clock::now();
 };
 
 namespace {
 
-// Completed batches: copy results out, release the slot and the matrix.
+// Forgets a request and returns its bytes to the request-memory budget.
+void erase_request(acc_server *s, std::unordered_map<uint64_t, Request>::iterator it) {
+    s->req_bytes -= it->second.bytes;
+    s->reqs.erase(it);
+}
+
+// Completed batches: copy results out, release the slot.
 void reap(acc_server *s) {
     for (int i = 0; i < s->o.slots; ++i) {
         Slot &sl = s->slots[i];
@@ -115,9 +128,9 @@ void reap(acc_server *s) {
             r.status = status;
             s->lat_us.push_back(std::chrono::duration<double, std::micro>(r.t1 - r.t0).count());
             ++s->completed;
-            if (!r.keep) s->reqs.erase(it);
+            if (!r.keep) erase_request(s, it);
         }
-        s->mats[(size_t)sl.matrix].inflight = false;
+        --s->mats[(size_t)sl.matrix].inflight;
         sl.busy = false;
         sl.ids.clear();
     }
@@ -132,7 +145,7 @@ void launch(acc_server *s) {
         Clock::time_point oldest = Clock::time_point::max();
         for (size_t mi = 0; mi < s->mats.size(); ++mi) {
             Matrix &mx = s->mats[mi];
-            if (!mx.m || mx.inflight || mx.pending.empty()) continue;
+            if (!mx.m || mx.pending.empty()) continue;
             const Clock::time_point t = s->reqs[mx.pending.front()].t0;
             if (t < oldest) {
                 oldest = t;
@@ -158,7 +171,7 @@ void launch(acc_server *s) {
         int rc = hipMemcpyAsync(sl.x_dev, sl.x_pin, (size_t)nq * mx.cols * 4, hipMemcpyHostToDevice, sl.st) == hipSuccess
                      ? ACC_OK : ACC_ERR_DEVICE;
         if (rc == ACC_OK)
-            rc = acc_gpu_topk_lowbit_batch(mx.m, sl.x_dev, nq, sl.ks.data(), s->o.ck, sl.o_dev, kmax, sl.st);
+            rc = acc::topk_lowbit_batch_ws(mx.m, sl.ws, sl.x_dev, nq, sl.ks.data(), s->o.ck, sl.o_dev, kmax, sl.st);
         if (rc == ACC_OK &&
             hipMemcpyAsync(sl.o_pin, sl.o_dev, (size_t)nq * kmax * sizeof(acc_element), hipMemcpyDeviceToHost, sl.st) != hipSuccess)
             rc = ACC_ERR_DEVICE;
@@ -173,13 +186,14 @@ void launch(acc_server *s) {
                 r.status = rc;
                 r.t1 = now;
                 ++s->completed;
-                if (!r.keep) s->reqs.erase(it);  // nobody waits for a load-generator request
+                if (!r.keep) erase_request(s, it);  // nobody waits for a load-generator request
             }
             sl.ids.clear();
             s->cv_done.notify_all();
             continue;
         }
-        mx.inflight = true;
+        ++mx.inflight;
+        s->peak_inflight = std::max<int64_t>(s->peak_inflight, mx.inflight);
         sl.busy = true;
         ++s->batches;
         s->batched += nq;
@@ -205,6 +219,7 @@ void worker_main(acc_server *s) {
 
 void free_slots(acc_server *s) {
     for (Slot &sl : s->slots) {
+        acc::batch_ws_destroy(sl.ws, s->o.device);
         if (sl.ev) hipEventDestroy(sl.ev);
         if (sl.st) hipStreamDestroy(sl.st);
         if (sl.x_pin) hipHostFree(sl.x_pin);
@@ -224,9 +239,11 @@ extern "C" int acc_server_create(const acc_server_opts *opts, acc_server **out) 
     if (o.slots == 0) o.slots = 4;
     if (o.ck == 0) o.ck = 512;
     if (o.max_pending == 0) o.max_pending = 1 << 16;
+    if (o.max_request_bytes == 0) o.max_request_bytes = int64_t(1) << 30;
     if (o.max_batch < 1 || o.max_batch > 1024 || o.slots < 1 || o.slots > kMaxSlots || o.ck < 1 || o.max_pending < 1 ||
-        o.memory_budget < 0)
-        return fail(ACC_ERR_ARG, "max_batch in [1, 1024], slots in [1, 8], ck >= 1, max_pending >= 1, budget >= 0");
+        o.memory_budget < 0 || o.max_request_bytes < 0)
+        return fail(ACC_ERR_ARG,
+                    "max_batch in [1, 1024], slots in [1, 8], ck >= 1, max_pending >= 1, budgets >= 0");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(ACC_ERR_NODEVICE, "no GPU visible");
     if (o.device < 0 || o.device >= ndev) return fail(ACC_ERR_ARG, "bad device index");
@@ -244,6 +261,7 @@ extern "C" int acc_server_create(const acc_server_opts *opts, acc_server **out) 
     if (!s->mem.reserve((int64_t)(xb + ob) * o.slots)) rc = fail(ACC_ERR_BUDGET, "memory budget below the slot buffers");
     for (int i = 0; i < o.slots && rc == ACC_OK; ++i) {
         Slot &sl = s->slots[i];
+        sl.ws = acc::batch_ws_create();
         if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||
             hipHostMalloc((void **)&sl.x_pin, xb, hipHostMallocDefault) != hipSuccess ||
@@ -296,35 +314,51 @@ extern "C" int acc_server_add_matrix(acc_server *s, const acc_csr *csr, const ac
         acc_gpu_matrix_destroy(m);
         return release(fail(ACC_ERR_BUDGET, "matrix does not fit in the server's memory budget"));
     }
-    const int warm[2] = {std::min(15, s->o.max_batch), s->o.max_batch};
-    float *x = nullptr;
-    acc_element *o = nullptr;
-    if (rc == ACC_OK && (hipMalloc((void **)&x, (size_t)s->o.max_batch * csr->cols * 4) != hipSuccess ||
-                         hipMalloc((void **)&o, (size_t)s->o.max_batch * sizeof(acc_element)) != hipSuccess))
-        rc = fail(ACC_ERR_NOMEM, "warm-up buffers");
-    if (rc == ACC_OK && hipMemset(x, 0, (size_t)s->o.max_batch * csr->cols * 4) != hipSuccess) rc = ACC_ERR_DEVICE;
-    for (int w = 0; w < 2 && rc == ACC_OK; ++w) {
-        std::vector<int32_t> ks((size_t)warm[w], 1);
-        rc = acc_gpu_topk_lowbit_batch(m, x, warm[w], ks.data(), s->o.ck, o, 1, nullptr);
-        if (rc == ACC_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(ACC_ERR_DEVICE, "warm-up batch failed");
-    }
-    if (x) hipFree(x);
-    if (o) hipFree(o);
     if (rc == ACC_OK && s->o.autotune)
         rc = acc_gpu_autotune(m, s->o.ck, std::min(100, std::min(kMaxK, s->o.ck)), s->o.max_batch, nullptr, 0, nullptr);
+    // the workspace a slot needs for this matrix: a scratch workspace warmed
+    // with batches of 1, 2, 4, ... max_batch queries (every layout the
+    // handle's batch-size table can pick), then freed
+    int64_t need = 0;
+    {
+        acc::BatchWorkspace *w = acc::batch_ws_create();
+        float *x = nullptr;
+        acc_element *o = nullptr;
+        if (rc == ACC_OK && (hipMalloc((void **)&x, (size_t)s->o.max_batch * csr->cols * 4) != hipSuccess ||
+                             hipMalloc((void **)&o, (size_t)s->o.max_batch * sizeof(acc_element)) != hipSuccess))
+            rc = fail(ACC_ERR_NOMEM, "warm-up buffers");
+        if (rc == ACC_OK && hipMemset(x, 0, (size_t)s->o.max_batch * csr->cols * 4) != hipSuccess) rc = ACC_ERR_DEVICE;
+        std::vector<int32_t> ks((size_t)s->o.max_batch, 1);
+        for (int b = 1; rc == ACC_OK; b = std::min(2 * b, s->o.max_batch)) {
+            rc = acc::topk_lowbit_batch_ws(m, w, x, b, ks.data(), s->o.ck, o, 1, nullptr);
+            if (rc == ACC_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(ACC_ERR_DEVICE, "warm-up batch failed");
+            if (b == s->o.max_batch) break;
+        }
+        need = acc::batch_ws_bytes(w);
+        acc::batch_ws_destroy(w, s->o.device);
+        if (x) hipFree(x);
+        if (o) hipFree(o);
+    }
     if (rc == ACC_OK) acc_gpu_matrix_info(m, &info);
-    // the reservation becomes the measured bytes, if they still fit
-    if (rc == ACC_OK && !s->mem.resize(reserved, info.device_bytes))
-        rc = fail(ACC_ERR_BUDGET, "matrix and its batch workspace exceed the budget");
     if (rc != ACC_OK) {
         acc_gpu_matrix_destroy(m);
         return release(rc);
     }
     std::lock_guard<std::mutex> lk(s->mu);
+    // the reservation becomes the measured bytes plus every slot's growth to
+    // this matrix's workspace, if they still fit (under the lock, so that
+    // concurrent additions see each other's workspace growth)
+    const int64_t grow = std::max<int64_t>(0, need - s->ws_need) * s->o.slots;
+    if (!s->mem.resize(reserved, info.device_bytes + grow)) {
+        acc_gpu_matrix_destroy(m);
+        s->mem.release(reserved);
+        return fail(ACC_ERR_BUDGET, "matrix and the slots' batch workspaces exceed the budget");
+    }
+    s->ws_need = std::max(s->ws_need, need);
     Matrix mx;
     mx.m = m;
     mx.cols = csr->cols;
-    mx.bytes = info.device_bytes;
+    mx.bytes = info.device_bytes + grow;
     s->mats.push_back(std::move(mx));
     *matrix_id = (int)s->mats.size() - 1;
     return ACC_OK;
@@ -341,8 +375,17 @@ static int submit(acc_server *s, int matrix_id, const float *x, int k, bool keep
         return fail(ACC_ERR_BUDGET, "request queue full");
     }
     Matrix &mx = s->mats[(size_t)matrix_id];
+    // host memory the request holds until it is read: the query, the
+    // results and its bookkeeping
+    const int64_t bytes = (int64_t)sizeof(Request) + 64 + (int64_t)mx.cols * 4 + (int64_t)k * (int64_t)sizeof(acc_element);
+    if (s->req_bytes + bytes > s->o.max_request_bytes) {
+        ++s->refused;
+        return fail(ACC_ERR_BUDGET, "request memory budget full");
+    }
     const uint64_t id = s->next_id++;
     Request &r = s->reqs[id];
+    r.bytes = bytes;
+    s->req_bytes += bytes;
     r.matrix = matrix_id;
     r.k = k;
     r.keep = keep;
@@ -370,10 +413,11 @@ extern "C" int acc_server_wait(acc_server *s, uint64_t ticket, acc_element *out,
     if (timeout_ms < 0) s->cv_done.wait(lk, ready);
     else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
         return fail(ACC_ERR_BUDGET, "timed out");
-    Request &r = s->reqs[ticket];
+    it = s->reqs.find(ticket);
+    Request &r = it->second;
     const int status = r.status;
     if (status == ACC_OK && out) std::copy(r.out.begin(), r.out.end(), out);
-    s->reqs.erase(ticket);
+    erase_request(s, it);
     return status == ACC_OK ? ACC_OK : fail(status, "request failed on the device");
 }
 
@@ -395,6 +439,8 @@ static void fill_stats(acc_server *s, acc_server_stats_t *st) {
     const double secs = std::chrono::duration<double>(Clock::now() - s->t_reset).count();
     st->qps = secs > 0 ? (double)s->completed / secs : 0.0;
     st->device_bytes = s->mem.used();
+    st->request_bytes = s->req_bytes;
+    st->peak_inflight = s->peak_inflight;
 }
 
 extern "C" int acc_server_stats(acc_server *s, acc_server_stats_t *st) {
@@ -409,6 +455,7 @@ extern "C" int acc_server_reset_stats(acc_server *s) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->lat_us.clear();
     s->submitted = s->completed = s->refused = s->batches = s->batched = 0;
+    s->peak_inflight = 0;
     s->t_reset = Clock::now();
     return ACC_OK;
 }

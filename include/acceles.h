@@ -381,9 +381,12 @@ void acc_gpu_plan_destroy(acc_plan *plan);
  * resident and forms batches from the waiting requests of each matrix (up to
  * max_batch, each request with its own K); batches of different matrices
  * run concurrently on up to `slots` streams, and requests that arrive while a
- * batch runs join the next batch of their matrix as soon as it can start.
- * Matrices are admitted against memory_budget (resident data plus the
- * workspace of a full batch: ACC_ERR_BUDGET otherwise), requests against
+ * batch runs join the next batch of their matrix as soon as a slot frees;
+ * every slot has its own batch workspace, so one matrix may have a batch on
+ * every slot.  Matrices are admitted against memory_budget (resident data
+ * plus every slot's workspace for a full batch of the matrix: ACC_ERR_BUDGET
+ * otherwise), requests against max_request_bytes (host memory held by a
+ * request until its result is read: the query, the results, bookkeeping) and
  * max_pending.  A result does not depend on the batch it lands in. */
 typedef struct acc_server acc_server;
 typedef struct {
@@ -394,6 +397,7 @@ typedef struct {
     int ck;                /* CK of every query (0 = 512)                           */
     int max_pending;       /* waiting requests before submissions are refused (0 = 65536) */
     int autotune;          /* 1: acc_gpu_autotune each matrix (K = 100) when it is added */
+    int64_t max_request_bytes; /* host bytes of waiting and unread requests (0 = 1 GiB) */
 } acc_server_opts;
 typedef struct {
     int64_t submitted, completed, refused, batches;
@@ -401,6 +405,8 @@ typedef struct {
     double p50_us, p99_us, max_us; /* latency from arrival to completion            */
     double qps;                /* completed per second since the last reset         */
     int64_t device_bytes;      /* held by the server                                */
+    int64_t request_bytes;     /* host bytes held by waiting and unread requests    */
+    int64_t peak_inflight;     /* most batches of one matrix in flight at once      */
 } acc_server_stats_t;
 int acc_server_create(const acc_server_opts *opts, acc_server **out);
 /* Uploads the matrix (FP32 CSR for rescoring + packets), runs a warm-up

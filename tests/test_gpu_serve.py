@@ -141,3 +141,58 @@ def test_concurrent_add_matrix_admission(acc, gpu):
     x = acc.generate_queries(1, 2048, seed=1)[0]
     assert len(srv.wait(srv.submit(ok[0], x, 10), 10)) == 10
     srv.close()
+
+
+def test_one_matrix_several_batches_in_flight(acc, gpu):
+    """Every slot has its own workspace: a burst against one matrix runs on several slots at once, and
+    each result equals the one the request gets alone and the oracle's."""
+    csr = acc.CSRMatrix.generate(200000, 500, deg=(40, 100), seed=17)
+    pk = csr.pack(bits=8, split=32)
+    srv = acc.Server(max_batch=16, slots=4, ck=512)
+    mid = srv.add_matrix(csr, pk)
+    X = acc.generate_queries(160, 500, seed=18)
+    ks = [[1, 8, 16, 32, 50, 75, 100][i % 7] for i in range(len(X))]
+    alone = [srv.wait(srv.submit(mid, X[i], ks[i]), ks[i]) for i in range(0, len(X), 20)]
+    srv.reset_stats()
+    tickets = [srv.submit(mid, X[i], ks[i]) for i in range(len(X))]
+    burst = [srv.wait(t, k) for t, k in zip(tickets, ks)]
+    st = srv.stats()
+    assert st["peak_inflight"] > 1, st
+    assert st["completed"] == len(X)
+    for j, i in enumerate(range(0, len(X), 20)):
+        np.testing.assert_array_equal(burst[i], alone[j])
+    ptr, idx, val = csr.arrays()
+    q, _ = O.quantize(val, 8, False)
+    gm = acc.GpuMatrix(csr, pk)
+    for i in range(0, len(X), 40):
+        _, oi, ov = O.two_stage(ptr, idx, val, q, X[i], 500, gm.info.group, 32, 512, ks[i])
+        np.testing.assert_array_equal(burst[i]["index"], oi)
+        np.testing.assert_array_equal(burst[i]["value"], ov)
+    srv.close()
+
+
+def test_requests_admitted_by_memory(acc, gpu):
+    """max_request_bytes bounds the host memory of waiting and unread requests: submissions past it are
+    refused with ACC_ERR_BUDGET, and reading results frees room again."""
+    csr = acc.CSRMatrix.generate(20000, 500, deg=(40, 100), seed=19)
+    pk = csr.pack(bits=8, split=32)
+    budget = 16 * (500 * 4 + 100 * 8 + 512)
+    srv = acc.Server(max_batch=8, slots=1, max_request_bytes=budget)
+    mid = srv.add_matrix(csr, pk)
+    x = acc.generate_queries(1, 500, seed=20)[0]
+    held, refused = [], 0
+    for _ in range(40):
+        try:
+            held.append(srv.submit(mid, x, 100))
+        except acc.AccelesError as e:
+            assert e.code == -8 and "request memory" in str(e)
+            refused += 1
+    # each request holds at least its query and its K results
+    assert 1 <= len(held) <= budget // (500 * 4 + 100 * 8) and refused == 40 - len(held) > 0
+    assert 0 < srv.stats()["request_bytes"] <= budget
+    first = srv.wait(held[0], 100)
+    for t in held[1:]:
+        np.testing.assert_array_equal(srv.wait(t, 100), first)
+    assert srv.stats()["request_bytes"] == 0
+    srv.wait(srv.submit(mid, x, 100), 100)  # room again
+    srv.close()
