@@ -149,14 +149,39 @@ def precision_at(got, exact):
     return [len(set(got[:k]) & set(exact[:k])) / k for k in KS]
 
 
+# the sources each profiled kernel is built from: a counter file counts only while they are unchanged
+KERNEL_SOURCES = {
+    "pk_scores": ["acceles_amd/csrc/k_pkmfma.hpp", "acceles_amd/csrc/k_pkmfma_a.hip", "acceles_amd/csrc/k_pkmfma_b.hip",
+                  "acceles_amd/csrc/k_pkmfma_c.hip", "acceles_amd/csrc/kernels_common.hpp",
+                  "acceles_amd/csrc/device_common.hpp"],
+    "lowbit_slicemax": ["acceles_amd/csrc/k_lowbit.hip", "acceles_amd/csrc/kernels_common.hpp",
+                        "acceles_amd/csrc/device_common.hpp"],
+}
+
+
+def sources_digest(kernel):
+    """sha256 over the kernel's source files (in KERNEL_SOURCES order)."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES[kernel]:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def pmc_traffic(kernel, rows, cols, bits):
     """HBM bytes per launch of `kernel` on this workload, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes
-    scaled by the same counters' reading for a copy kernel of known size; None when not profiled."""
+    scaled by the same counters' reading for a copy kernel of known size; None when not profiled, or when
+    the file was measured on other kernel sources (its sources_sha256 differs from this tree's)."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{rows}x{cols}_{bits}bit_pmc.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        d = json.load(f)
+    if kernel in KERNEL_SOURCES and d.get("sources_sha256") != sources_digest(kernel):
+        log(f"[bench] {os.path.basename(path)}: measured on other sources of {kernel}; traffic not reported")
+        return None
+    return d.get("hbm_bytes_per_launch")
 
 
 def line_common(a, metric, value, unit, ms, dtype, workload, extra_cfg):
