@@ -726,12 +726,14 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         std::vector<uint32_t> p32(csr->ptr.size());
         for (size_t i = 0; i < p32.size(); ++i) p32[i] = (uint32_t)csr->ptr[i];
         TRY(dalloc(m, &m->d_ptr, p32.size()));
-        TRY(dalloc(m, &m->d_idx, (size_t)nnz));
-        TRY(dalloc(m, &m->d_val, (size_t)nnz));
+        TRY(dalloc(m, &m->d_idx, (size_t)nnz + 4));  // 4 entries of padding for fp32_spmv's 16-byte loads
+        TRY(dalloc(m, &m->d_val, (size_t)nnz + 4));
         TRY(dalloc(m, &m->d_y, (size_t)m->rows));
         COPY(m->d_ptr, p32.data(), p32.size() * 4);
         COPY(m->d_idx, csr->idx.data(), (size_t)nnz * 4);
         COPY(m->d_val, csr->val.data(), (size_t)nnz * 4);
+        ZERO(m->d_idx + nnz, 16);
+        ZERO(m->d_val + nnz, 16);
         select_geometry(m->rows, m->cus, m->sel_blocks, m->sel_chunk);
         TRY(dalloc(m, &m->d_sthr, 1));
         TRY(dalloc(m, &m->d_scnt, 1));
@@ -1013,14 +1015,21 @@ extern "C" int acc_gpu_spmv_fp64(acc_gpu_matrix *m, const float *x_dev, double *
 // FP32 CSR path
 // ---------------------------------------------------------------------------
 
+size_t fp32_spmv_lds(int cols, int G);  // k_fp32.hip
+
 static int launch_spmv_fp32(acc_gpu_matrix *m, const float *x_dev, float *y, hipStream_t st) {
     spmv_fn k = get_fp32_spmv(m->group);
     if (!k) return fail(ACC_ERR_ARG, "unsupported group size");
     const int rpi = 64 / m->group;
     const int nblocks = (m->rows + rpi - 1) / rpi;
-    const int blocks = std::max(1, std::min((nblocks + ACC_WAVES * 4 - 1) / (ACC_WAVES * 4), m->cus * 8));
-    LAUNCH(m, 0, k, dim3(blocks), dim3(ACC_WG), (size_t)m->cols * 4, st, m->d_ptr, m->d_idx, m->d_val, x_dev, m->rows,
-           m->cols, y);
+    // as many workgroups as run at once (each wavefront then owns one contiguous row range, and no
+    // second wave of workgroups leaves part of the chip idle at the end), fewer for small matrices
+    const size_t lds = fp32_spmv_lds(m->cols, m->group);
+    int occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k, ACC_WG, lds) != hipSuccess || occ < 1)
+        occ = 1;
+    const int blocks = std::max(1, std::min((nblocks + ACC_WAVES * 4 - 1) / (ACC_WAVES * 4), m->cus * occ));
+    LAUNCH(m, 0, k, dim3(blocks), dim3(ACC_WG), lds, st, m->d_ptr, m->d_idx, m->d_val, x_dev, m->rows, m->cols, y);
     return check_launch();
 }
 

@@ -15,15 +15,30 @@
 // y = A x with G lanes per row.  Lane g of a row sums its entries
 // g, g+G, g+2G, ... in that order with fmaf, then an xor butterfly of plain
 // adds.  Each wavefront owns a contiguous range of rows and takes B blocks of
-// 64/G rows per step: the first U entries per lane of all B blocks are loaded
-// together (rows longer than U*G entries continue block by block), so a step
-// keeps B times more loads in flight than one block at a time.  The next
-// step's row bounds are fetched early.  x is staged in LDS.
+// 64/G rows per step.  The step's rows hold one contiguous range of entries:
+// for short rows (G <= 8) and when it fits kFp32Stage entries, it is first
+// copied into the wavefront's LDS area with 16-byte loads (whole cache lines,
+// each byte read once), one step ahead, and the G-lane order then reads LDS;
+// otherwise the lanes load the entries directly, the
+// first U per lane of all B blocks together (rows longer than U*G entries
+// continue block by block).  The next step's bounds are fetched early.  x is
+// staged in LDS.  idx and val hold 4 entries of padding past nnz (the last
+// 16-byte load may reach 3 entries past the range).
+static constexpr int kFp32Stage = 1024;  // entries per wavefront staged in LDS (value and column: 8 KB)
+
+// LDS of an fp32_spmv<G> workgroup: x, then (G <= 8) each wavefront's staging area.
+size_t fp32_spmv_lds(int cols, int G) {
+    return (((size_t)cols * 4 + 15) & ~(size_t)15) + (G <= 8 ? (size_t)ACC_WAVES * kFp32Stage * 8 : 0);
+}
+
 template <int G, int B>
 __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const int32_t *idx, const float *val,
                                                    const float *x, int rows, int cols, float *y) {
     extern __shared__ __align__(16) unsigned char smem[];
     float *xs = (float *)smem;
+    const uint32_t xs_bytes = ((uint32_t)cols * 4u + 15u) & ~15u;
+    float *sv = (float *)(smem + xs_bytes) + (threadIdx.x >> 6) * kFp32Stage;  // this wavefront's staged values
+    int32_t *sc = (int32_t *)(smem + xs_bytes + ACC_WAVES * kFp32Stage * 4) + (threadIdx.x >> 6) * kFp32Stage;
     for (int c0 = 0; c0 < cols; c0 += 4 * ACC_WG) {  // four loads per thread issued together
         float t[4];
 #pragma unroll
@@ -54,6 +69,38 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
         jb[t] = ok ? p0 : 0u;
         je[t] = ok ? p1 : 0u;
     }
+    // the entry range of a step: its first row's start to its last row's end
+    auto range = [&](int b, uint32_t &s, uint32_t &e) {
+        const int r0 = b * RPI, r1 = min(min(b + B, b1) * RPI, rows);
+        s = ptr[r0];
+        e = ptr[r1];
+    };
+    // the step's entries in flight in registers: 16-byte loads over [s & ~3, e); rows of up to 8 lanes
+    // only (longer rows already read whole cache lines per lane group)
+    constexpr bool kStaged = G <= 8;
+    constexpr int NL = kFp32Stage / 256;
+    typedef float vf4 __attribute__((ext_vector_type(4)));
+    typedef int vi4 __attribute__((ext_vector_type(4)));
+    vf4 lv[NL];
+    vi4 lc[NL];
+    auto issue = [&](uint32_t s, uint32_t e) {
+        const uint32_t sa = s & ~3u, cnt = e - sa;
+        if (!kStaged || cnt > (uint32_t)kFp32Stage) return false;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const uint32_t o = (uint32_t)(i * 256 + lane * 4);
+            if (o < cnt) {
+                lv[i] = __builtin_nontemporal_load((const vf4 *)(val + sa + o));
+                lc[i] = __builtin_nontemporal_load((const vi4 *)(idx + sa + o));
+            }
+        }
+        return true;
+    };
+    // ranges: this step's (rs, re), the next one's (ns, ne2); the one after is fetched in the loop
+    uint32_t rs, re, ns = 0, ne2 = 0;
+    range(b0, rs, re);
+    if (b0 + B < b1) range(b0 + B, ns, ne2);
+    bool have = issue(rs, re);  // this step's entries are in lv/lc
     for (int b = b0; b < b1; b += B) {
         uint32_t nb[B], ne[B];
 #pragma unroll
@@ -64,6 +111,61 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
             const uint32_t p0 = ptr[rc], p1 = ptr[rc + 1];
             nb[t] = ok ? p0 : 0u;
             ne[t] = ok ? p1 : 0u;
+        }
+        uint32_t ms = 0, me = 0;
+        if (b + 2 * B < b1) range(b + 2 * B, ms, me);
+        if (have) {
+            const uint32_t sa = rs & ~3u, cnt = re - sa;
+#pragma unroll
+            for (int i = 0; i < NL; ++i) {
+                const uint32_t o = (uint32_t)(i * 256 + lane * 4);
+                if (o < cnt) {
+                    *(vf4 *)(sv + o) = lv[i];
+                    *(vi4 *)(sc + o) = lc[i];
+                }
+            }
+            wave_sync();
+            // the next step's entries fly while this one is summed from LDS
+            const bool next = b + B < b1 && issue(ns, ne2);
+            // the first U entries per lane of all B blocks: their LDS reads issued together (clamped
+            // into the staged range, masked after), then longer rows entry by entry
+            const uint32_t last = cnt ? cnt - 1u : 0u;  // clamped reads stay in the staged range
+            float p[B], a[B][U], xv[B][U];
+#pragma unroll
+            for (int t = 0; t < B; ++t)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t o = min(jb[t] + g + u * G - sa, last);
+                    a[t][u] = sv[o];
+                    xv[t][u] = xs[min((uint32_t)sc[o], (uint32_t)cols - 1u)];
+                }
+#pragma unroll
+            for (int t = 0; t < B; ++t) {
+                p[t] = 0.f;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (jb[t] + g + u * G < je[t]) p[t] = __builtin_fmaf(a[t][u], xv[t][u], p[t]);
+                for (uint32_t jj = jb[t] + g + U * G; jj < je[t]; jj += G) p[t] = __builtin_fmaf(sv[jj - sa], xs[sc[jj - sa]], p[t]);
+            }
+#pragma unroll
+            for (int t = 0; t < B; ++t) {
+#pragma unroll
+                for (int s2 = G / 2; s2 >= 1; s2 >>= 1) p[t] = p[t] + __shfl_xor(p[t], s2);
+                const int r = (b + t) * RPI + lane / G;
+                if (g == 0 && b + t < b1 && r < rows) y[r] = p[t];
+            }
+            wave_sync();  // the staged entries are read before the next step overwrites them
+#pragma unroll
+            for (int t = 0; t < B; ++t) {
+                jb[t] = nb[t];
+                je[t] = ne[t];
+            }
+            rs = ns;
+            re = ne2;
+            ns = ms;
+            ne2 = me;
+            have = next;
+            continue;
         }
         float v[B][U];
         int c[B][U];
@@ -108,6 +210,11 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
             jb[t] = nb[t];
             je[t] = ne[t];
         }
+        have = b + B < b1 && issue(ns, ne2);
+        rs = ns;
+        re = ne2;
+        ns = ms;
+        ne2 = me;
     }
 }
 
