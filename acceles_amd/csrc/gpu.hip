@@ -29,6 +29,7 @@ static constexpr int kMaxCK = 4096;  // candidates of the low-bit path
 static constexpr size_t kWordPad = 32 * 64;  // dwords past the packets (>= widest group, 22 x 64)
 static constexpr int kPkMaxCols = 1024;  // matrix-core scores: B operands of 32 column blocks in registers
 static constexpr int kPkMaxE = 18;       // entry widths instantiated for them
+static constexpr size_t kPkLdsMax = 160 * 1024;  // LDS of a pk_scores workgroup at most
 
 // Per-batch device workspace of the batched path.  The handle owns one;
 // every plan (captured graph) owns its own, so replaying a plan never touches
@@ -394,6 +395,7 @@ DenseArgs dense_args(const acc_gpu_matrix *m) {
     a.is_signed = m->is_signed;
     a.u8off = m->u8off;
     a.dup = m->dup ? 1 : 0;
+    a.nimg = 2;
     a.split = m->split;
     a.ptr = m->d_ptr;
     a.idx = m->d_idx;
@@ -470,7 +472,10 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     if ((rc = check_launch()) != ACC_OK) return rc;
     dense_fn fs = pk_kernel(m);
     if (!fs) return fail(ACC_ERR_ARG, "unsupported entry width");
-    const size_t lds_s = (size_t)2 * 64 * (m->cols_pad + 16) + kPkLdsTail;  // two slice images and the tail
+    // three slice images when they fit (the decode runs a round ahead, one barrier per slice), else two
+    const size_t img = (size_t)64 * (m->cols_pad + 16);
+    a.nimg = 3 * img + kPkLdsTail <= kPkLdsMax ? 3 : 2;
+    const size_t lds_s = (size_t)a.nimg * img + kPkLdsTail;
     if ((rc = lds_limit((const void *)fs, lds_s)) != ACC_OK) return rc;
     // workgroups per query block: as many as fit the chip at once, a multiple
     // of 8 (the kernel keeps a slice range's query blocks on one XCD)

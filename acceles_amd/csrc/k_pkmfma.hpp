@@ -130,6 +130,7 @@ struct PkCtx {
     int lane, hi, wave;
     int64_t end;  // one past this workgroup's last slice
     uint32_t cols_pad, stride, img_bytes, fill, cmask, row_base, a_base;
+    uint32_t nimg;  // slice images in LDS: 2 (fill, barrier, decode per round) or 3 (fill two rounds ahead)
     int cb, vb;
     __amdgpu_buffer_rsrc_t meta;  // a.slices
 };
@@ -138,7 +139,8 @@ struct PkCtx {
 // a compile-time step, so the B operands and accumulators are addressed by
 // constants.
 template <int KB, int P, bool LISTS, int kb>
-__device__ __forceinline__ void pk_mstep(PkMRegs &r, const pv4i (&B)[KB], const pv4i *A0, const pv4i *A1) {
+__device__ __forceinline__ void pk_mstep(const PkCtx &c, PkMRegs &r, const pv4i (&B)[KB], const pv4i *A0,
+                                         const pv4i *A1) {
     constexpr int RING = 4, DIST = 2;  // A operands: ring slots, column blocks ahead
     constexpr int ZS = KB / 4;         // the decode wavefronts' fill barrier
     if constexpr (kb + DIST < KB) {    // constant offsets; blocks past nkb meet zero B operands
@@ -148,7 +150,9 @@ __device__ __forceinline__ void pk_mstep(PkMRegs &r, const pv4i (&B)[KB], const 
     __builtin_amdgcn_sched_barrier(0);
     r.acc[P][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(r.ar[kb % RING][0], B[kb], r.acc[P][0], 0, 0, 0);
     r.acc[P][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(r.ar[kb % RING][1], B[kb], r.acc[P][1], 0, 0, 0);
-    if constexpr (kb == ZS) lds_barrier();
+    if constexpr (kb == ZS) {
+        if (c.nimg == 2) lds_barrier();  // two images: the next one's fill is complete
+    }
     // the previous slice's accumulators into the list, value v at step v*KB/32
 #pragma unroll
     for (int v = 0; v < 32; ++v)
@@ -158,9 +162,9 @@ __device__ __forceinline__ void pk_mstep(PkMRegs &r, const pv4i (&B)[KB], const 
 }
 
 template <int KB, int P, bool LISTS, int... KBS>
-__device__ __forceinline__ void pk_mchain(std::integer_sequence<int, KBS...>, PkMRegs &r, const pv4i (&B)[KB],
-                                          const pv4i *A0, const pv4i *A1) {
-    (pk_mstep<KB, P, LISTS, KBS>(r, B, A0, A1), ...);
+__device__ __forceinline__ void pk_mchain(std::integer_sequence<int, KBS...>, const PkCtx &c, PkMRegs &r,
+                                          const pv4i (&B)[KB], const pv4i *A0, const pv4i *A1) {
+    (pk_mstep<KB, P, LISTS, KBS>(c, r, B, A0, A1), ...);
 }
 
 // Lanes 0..31 receive the value of lane + 32 (v_permlane32_swap; lanes
@@ -188,13 +192,13 @@ __device__ __forceinline__ void pk_store_list(const PkCtx &c, uint32_t Dk, int q
     if (c.hi == 0 && q < a.nq) *(uint4 *)(a.tk + ((size_t)q * a.nslices + sx) * 4) = make_uint4(k0, k1, k2, k3);
 }
 
-// MFMA round of parity P: slice s (image P) multiplied into acc[P]; the list
-// of the previous slice (acc[1-P], none for the first slice: !LISTS)
-// reduced during the chain and stored.
+// MFMA round of parity P: the slice in image `img` multiplied into acc[P];
+// the list of the previous slice (acc[1-P], none for the first slice:
+// !LISTS) reduced during the chain and stored.
 template <int KB, int P, bool LISTS>
 __device__ __forceinline__ void pk_mround(const PkCtx &c, PkMRegs &r, const pv4i (&B)[KB], uint32_t Dk, int q,
-                                          int64_t s_prev) {
-    const unsigned char *cur = c.smem + (P ? c.img_bytes : 0u);
+                                          int64_t s_prev, uint32_t img) {
+    const unsigned char *cur = c.smem + img * c.img_bytes;
     r.acc[P][0] = (pv16i){0};
     r.acc[P][1] = (pv16i){0};
 #pragma unroll
@@ -206,7 +210,7 @@ __device__ __forceinline__ void pk_mround(const PkCtx &c, PkMRegs &r, const pv4i
         r.ar[j][0] = A0[j * 2];
         r.ar[j][1] = A1[j * 2];
     }
-    pk_mchain<KB, P, LISTS>(std::make_integer_sequence<int, KB>(), r, B, A0, A1);
+    pk_mchain<KB, P, LISTS>(std::make_integer_sequence<int, KB>(), c, r, B, A0, A1);
     if constexpr (LISTS) pk_store_list(c, Dk, q, r.kk[0], r.kk[1], r.kk[2], r.kk[3], s_prev);
     lds_barrier();  // this slice's image read, the next one written
 }
@@ -236,9 +240,12 @@ __device__ __forceinline__ void pk_mmain(const PkCtx &c, int qb, int tile, int64
     const int q = qb * a.NT * 32 + tile * 32 + (c.lane & 31);
     const uint32_t Dk = ((uint32_t)(a.corr[q] + (1 << 25)) << 6) + (uint32_t)(c.hi * 4);
     PkMRegs r;
-    lds_barrier();  // both images filled
+    lds_barrier();  // the images filled
     lds_barrier();  // the first slice decoded
-    pk_mround<KB, 0, false>(c, r, B, Dk, q, -1);
+    // slice i of the range lies in image i % nimg
+    uint32_t img = 0;
+    auto next_img = [&]() { img = img + 1 == c.nimg ? 0u : img + 1; };
+    pk_mround<KB, 0, false>(c, r, B, Dk, q, -1, img);
     int64_t s_prev = s;
     ++s;
     for (;;) {
@@ -246,13 +253,15 @@ __device__ __forceinline__ void pk_mmain(const PkCtx &c, int qb, int tile, int64
             pk_last_list<0>(c, r, Dk, q, s_prev);
             return;
         }
-        pk_mround<KB, 1, true>(c, r, B, Dk, q, s_prev);
+        next_img();
+        pk_mround<KB, 1, true>(c, r, B, Dk, q, s_prev, img);
         s_prev = s++;
         if (s >= c.end) {
             pk_last_list<1>(c, r, Dk, q, s_prev);
             return;
         }
-        pk_mround<KB, 0, true>(c, r, B, Dk, q, s_prev);
+        next_img();
+        pk_mround<KB, 0, true>(c, r, B, Dk, q, s_prev, img);
         s_prev = s++;
     }
 }
@@ -289,21 +298,27 @@ __device__ __forceinline__ uint2 pk_slice_meta(const PkCtx &c, int64_t s) {
     return m;
 }
 
-// Decode round of parity P (the MFMA wavefronts multiply slice s from image
-// P): image 1-P filled, then slice s + 1 (metadata sl1) decoded into it --
-// group dw from the registers w[P], requested a round earlier, and groups dw
-// + kPkDecodeWaves, ... loaded here; group dw of slice s + 2 (sl2)
-// requested into w[1-P].  sl1, sl2 advance by one slice.
+// Decode round of parity P (the MFMA wavefronts multiply slice s): slice
+// s + 1 (metadata sl1) decoded into image `nx` -- group dw from the registers
+// w[P], requested a round earlier, and groups dw + kPkDecodeWaves, ...
+// loaded here; group dw of slice s + 2 (sl2) requested into w[1-P].  With
+// two images, image nx is filled first (then a barrier); with three, image
+// `nf` (slice s + 2's) is filled for the next round and image nx was filled
+// in the previous one.  sl1, sl2 advance by one slice.
 template <int E, int VM, bool DEC, int P>
 __device__ __forceinline__ void pk_dround(const PkCtx &c, uint32_t (&w)[2][E], int dw, int64_t s, uint2 &sl1,
-                                          uint2 &sl2) {
+                                          uint2 &sl2, uint32_t nx, uint32_t nf) {
     const DenseArgs &a = *c.a;
-    unsigned char *nxt = c.smem + (P ? 0u : c.img_bytes);
+    unsigned char *nxt = c.smem + nx * c.img_bytes;
     const int64_t s3 = s + 3;
     const uint2 s3m = pk_slice_meta(c, s3);
     if constexpr (DEC) load_group<E>(w[1 - P], slice_rsrc(a.words, sl2), dw * E);
-    pk_fill(c, nxt, c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
-    lds_barrier();  // the image holds the background
+    if (c.nimg == 2) {
+        pk_fill(c, nxt, c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
+        lds_barrier();  // the image holds the background
+    } else {
+        pk_fill(c, c.smem + nf * c.img_bytes, c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
+    }
     if constexpr (DEC) {
         if (dw * E < (int)sl1.y) {
 #pragma unroll
@@ -326,23 +341,32 @@ template <int E, int VM, bool DEC>
 __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
     const DenseArgs &a = *c.a;
     uint32_t w[2][E];
-    pk_fill(c, c.smem, 2u * c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
+    pk_fill(c, c.smem, c.nimg * c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
     const uint2 s0m = pk_slice_meta(c, s);
     const uint2 s1m = pk_slice_meta(c, s + 1), s2m = pk_slice_meta(c, s + 2);
     uint2 sl1 = s + 1 < c.end ? s1m : make_uint2(0u, 0u);
     uint2 sl2 = s + 2 < c.end ? s2m : make_uint2(0u, 0u);
     if constexpr (DEC) load_group<E>(w[0], slice_rsrc(a.words, sl1), dw * E);
-    lds_barrier();  // both images filled
+    lds_barrier();  // the images filled
     if constexpr (DEC) pk_decode_groups<E, VM, false>(c, s0m, dw, kPkDecodeWaves, c.smem);
     else if (dw == 0) pk_decode_groups<E, VM, true>(c, s0m, 0, 1, c.smem);
     lds_barrier();  // the first slice decoded
-    pk_dround<E, VM, DEC, 0>(c, w, dw, s, sl1, sl2);
+    // slice i of the range lies in image i % nimg: round i decodes slice i + 1
+    // (image nx) and, with three images, fills slice i + 2's (image nf)
+    uint32_t nx = 1u % c.nimg, nf = 2u % c.nimg;
+    auto advance = [&]() {
+        nx = nx + 1 == c.nimg ? 0u : nx + 1;
+        nf = nf + 1 == c.nimg ? 0u : nf + 1;
+    };
+    pk_dround<E, VM, DEC, 0>(c, w, dw, s, sl1, sl2, nx, nf);
     ++s;
     for (;;) {
         if (s >= c.end) return;
-        pk_dround<E, VM, DEC, 1>(c, w, dw, s, sl1, sl2);
+        advance();
+        pk_dround<E, VM, DEC, 1>(c, w, dw, s, sl1, sl2, nx, nf);
         if (++s >= c.end) return;
-        pk_dround<E, VM, DEC, 0>(c, w, dw, s, sl1, sl2);
+        advance();
+        pk_dround<E, VM, DEC, 0>(c, w, dw, s, sl1, sl2, nx, nf);
         ++s;
     }
 }
@@ -386,6 +410,7 @@ __attribute__((amdgpu_waves_per_eu(PkShape::kWavesPerEU, PkShape::kWavesPerEU)))
     c.cols_pad = (uint32_t)a.cols_pad;
     c.stride = c.cols_pad + 16u;
     c.img_bytes = 64u * c.stride;
+    c.nimg = (uint32_t)a.nimg;
     c.fill = VM == PK_U8 ? 0x80808080u : 0u;
     c.cmask = (1u << a.col_bits) - 1u;
     c.row_base = (uint32_t)lane * c.stride;                            // scatter: lane = row

@@ -632,6 +632,7 @@ struct DenseArgs {
     const uint8_t *nvalid;       // valid rows per slice (a prefix of its lanes)
     int64_t nslices;
     int cols, cols_pad, nkb, col_bits, val_bits, is_signed, u8off, dup;
+    int nimg;                    // pk_scores: slice images in LDS (2 or 3)
     const float *X;              // nq x cols FP32 queries
     int nq, NT, nqb;             // queries; 32-query tiles per block; blocks
     int8_t *qtab;                // nqb x nkb x NT x 1024 B-operand bytes
