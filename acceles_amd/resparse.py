@@ -59,23 +59,39 @@ def search_keep_ratio(csr, bits=8, split=32, ck=512, nq=100, accuracy_threshold=
             prec.append([len(set(exact[i][:k].tolist()) & cand) / k for k in KS])
         return convergence(prec), m.nnz
 
-    history = []
-    c0, nnz0 = measure(1.0)
-    history.append((1.0, c0, nnz0))
-    threshold = c0 * accuracy_threshold
+    def logged(keep):
+        c, nnz = measure(keep)
+        if log:
+            log(f"resparse keep {keep:.3f}: convergence {c:.5f}, nnz {nnz}")
+        return c, nnz
+
+    return bisect_keep(logged, accuracy_threshold, max_epochs)
+
+
+def bisect_keep(measure, accuracy_threshold, max_epochs=100):
+    """The keep-ratio bisection of search_best_resparse_ratio.py:55-100 (both the embedding and the
+    generated-matrix scripts run it): measure(keep) returns (convergence, ...); epoch 0 measures keep = 1
+    and sets the threshold to accuracy_threshold times its convergence; epoch e >= 1 measures keep
+    (0.5 first), moves `high` to keep when the convergence reaches the threshold and `low` otherwise,
+    and the next keep is round((low + high) / 2, 3); the search stops when that value was measured
+    before, or after max_epochs epochs.  Returns {"keep": high, "threshold", "history": [(keep, *measure
+    (keep))], "converged": whether it stopped on a repeated value}."""
+    first = tuple(measure(1.0))
+    history = [(1.0,) + first]
+    threshold = first[0] * accuracy_threshold
     low, high, keep = 0.0, 1.0, 0.5
     tried = {1.0}
+    converged = False
     for _ in range(1, max_epochs):
         tried.add(keep)
-        c, nnz = measure(keep)
-        history.append((keep, c, nnz))
-        if log:
-            log(f"resparse keep {keep:.3f}: convergence {c:.5f} (threshold {threshold:.5f}), nnz {nnz}")
-        if c >= threshold:
+        r = tuple(measure(keep))
+        history.append((keep,) + r)
+        if r[0] >= threshold:
             high = keep
         else:
             low = keep
         keep = round((low + high) / 2, 3)
         if keep in tried:
+            converged = True
             break
-    return {"keep": high, "threshold": threshold, "history": history}
+    return {"keep": high, "threshold": threshold, "history": history, "converged": converged}

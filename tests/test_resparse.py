@@ -83,3 +83,74 @@ def test_library_rejects_bad_ratio(acc):
     csr = acc.CSRMatrix.generate(10, 8, deg=(1, 3), seed=1)
     with pytest.raises(acc.AccelesError, match="keep"):
         csr.resparse(1.5)
+
+
+# --- the keep-ratio search against the reference's own search logs ---------------------------------
+
+def search_logs():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "resparse_search_logs.json")) as f:
+        return json.load(f)
+
+
+def logged_convergence(e):
+    """An epoch's convergence: the value the log printed, else the metric of its printed precisions."""
+    from acceles_amd.resparse import convergence
+    if e["convergence"] is not None:
+        return e["convergence"]
+    return convergence([[p / 100 for p in e["precision_pct"]]])
+
+
+def test_convergence_metric_matches_the_logged_values():
+    """convergence() of each epoch's printed mean precisions is the convergence the reference printed
+    (quantify_util.py:433-436; self_generated_data/csr_util.py:260): exactly up to rounding where the
+    precisions are printed unrounded, within the 0.005 % print rounding (x sum of weights) otherwise."""
+    from acceles_amd.resparse import WEIGHTS, convergence
+    n = 0
+    for log in search_logs():
+        for e in log["epochs"]:
+            if e["convergence"] is None or len(e["precision_pct"]) != 7:
+                continue
+            got = convergence([[p / 100 for p in e["precision_pct"]]])
+            tol = 1e-12 if not e["rounded"] else 0.5e-4 * sum(WEIGHTS) + 1e-12
+            assert abs(got - e["convergence"]) <= tol, (log["log"], e["epoch"], got, e["convergence"])
+            n += 1
+    assert n >= 150
+    # a dropped weight or a transposed one is far outside that tolerance
+    e = search_logs()[6]["epochs"][1]
+    p = [x / 100 for x in e["precision_pct"]]
+    assert abs(convergence([p[::-1]]) - e["convergence"]) > 0.1
+
+
+def test_bisection_replays_every_logged_search():
+    """bisect_keep fed each epoch's logged convergence tries the logged ratios in the logged order and
+    settles where the reference did ("Convergence reached ...%" prints high * 100)."""
+    from acceles_amd.resparse import bisect_keep
+    for log in search_logs():
+        by_keep = {round(e["top_percent"] / 100, 3): logged_convergence(e) for e in log["epochs"]}
+        tried = []
+
+        def measure(keep):
+            tried.append(keep)
+            return (by_keep[keep],)
+
+        n = len(log["epochs"])
+        res = bisect_keep(measure, log["accuracy_threshold"], max_epochs=n)
+        assert [round(k * 100, 1) for k in tried] == [e["top_percent"] for e in log["epochs"]], log["log"]
+        assert res["threshold"] == log["epochs"][0]["convergence"] * log["accuracy_threshold"]
+        if log["converged_percent"] is None:  # a log that ends before the search stopped
+            assert not res["converged"]
+        else:
+            assert res["converged"] and res["keep"] * 100 == log["converged_percent"], log["log"]
+
+
+def test_glove_search_settles_at_the_published_ratio():
+    """The GloVe 6B search at threshold 0.999, CK=512, SPLIT=32 keeps 52.9 % (precision@100 96.61 %)."""
+    from acceles_amd.resparse import bisect_keep
+    log = next(l for l in search_logs() if "glove.6B.50d.txt_search_resparse_0.999_" in l["log"])
+    by_keep = {round(e["top_percent"] / 100, 3): logged_convergence(e) for e in log["epochs"]}
+    res = bisect_keep(lambda k: (by_keep[k],), 0.999)
+    assert res["keep"] == 0.529 and len(res["history"]) == 11
+    last = log["epochs"][-1]
+    assert last["top_percent"] == 52.9 and last["precision_pct"][-1] == 96.61
