@@ -175,22 +175,23 @@ __device__ __forceinline__ void list_rows(const DenseArgs &a, const uint4 (&k4)[
 // slice whose list may have dropped such a row is rescanned exactly (from its
 // packets).  Keys of padding rows are discarded, and since they may have
 // displaced real rows from a list, every list whose fourth key scores >= ts
-// is rescanned.
+// is rescanned.  Rows are looked up for the offered keys only.
 template <int U>
 __device__ __forceinline__ void offer_lists(const DenseArgs &a, WaveTopM<2> &cb, const uint4 (&k4)[U], const int (&nv)[U],
-                                   const int32_t (&row)[U][4], int64_t s0, int b, int32_t ts, int q, int8_t *qx,
-                                   bool &have_qx) {
+                                   int64_t s0, int b, int32_t ts, int q, int8_t *qx, bool &have_qx) {
     const int lane = lane_id();
     bool rescan[U], pass[U * 4];
     uint64_t key[U * 4];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         rescan[u] = nv[u] > 0 && tk_score(k4[u].w) >= ts;
+        const int64_t s = min(s0 + b + u * 64 + lane, a.nslices - 1);
         const uint32_t ke[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            pass[u * 4 + e] = !rescan[u] && row[u][e] >= 0 && tk_score(ke[e]) >= ts;
-            key[u * 4 + e] = dkey_i32(tk_score(ke[e]), row[u][e]);
+            pass[u * 4 + e] = !rescan[u] && tk_rho(ke[e]) < nv[u] && tk_score(ke[e]) >= ts;
+            const int32_t r = pass[u * 4 + e] ? a.perm[s * 64 + tk_rho(ke[e])] : 0;
+            key[u * 4 + e] = dkey_i32(tk_score(ke[e]), r);
         }
     }
     cb.push_many<U * 4>(pass, key);
@@ -297,7 +298,9 @@ __device__ __forceinline__ void emit_candidates(const DenseArgs &a, const uint64
 //     the lower-row tie rule), then their FP32 rescoring (out != null).
 // Partitions of at most 256 slices (M <= 32, at most 64 candidates) keep
 // their lists in registers: one prefix sum appends the candidates, a count
-// ranks them.  Anything else streams the lists twice through a collector.
+// ranks them.  Up to 1024 slices the lists are still read once into
+// registers, and offered to a collector; anything larger streams them twice
+// (threshold, then offers).
 __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x >> 6, lane = lane_id();
@@ -442,22 +445,48 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
         }
     }
 
-    // General path: the lists are streamed twice (threshold, then offers).
+    // General path: a collector of the offered keys.
     if (a.out) stage_query(xs, xq, a.cols);
     WaveTopM<2> cb;
     cb.init(bufB, M);
     constexpr int U = 4;
     int32_t ts = INT32_MIN;
-    if (nsl <= 1024 && M <= 32) {  // the M-th largest maximum by extraction, maxima in registers
+    if (nsl <= 1024 && M <= 32) {
+        // every list in registers (16 per lane, loaded together, clamped into
+        // the partition and masked after); the threshold is the M-th largest
+        // full-slice maximum, then the lists are offered from the registers
+        uint4 k4[4][U];
+        int nv[4][U];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = min((j * U + u) * 64 + lane, nsl - 1);
+                k4[j][u] = L[i];
+                nv[j][u] = (int)a.nvalid[s0 + i];
+            }
         uint32_t v[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int i = u * 64 + lane;
-            v[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
-        }
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if ((j * U + u) * 64 + lane >= nsl) {
+                    k4[j][u] = make_uint4(0u, 0u, 0u, 0u);
+                    nv[j][u] = 0;
+                }
+                v[j * U + u] = nv[j][u] == 64 ? k4[j][u].x : 0u;
+            }
         const uint32_t t = wave_kth_u32<16>(v, M);
         if (t != 0u) ts = tk_score(t);
-    } else {
+        offer_lists<U>(a, cb, k4[0], nv[0], s0, 0, ts, q, qx, have_qx);
+        if (U * 64 < nsl) offer_lists<U>(a, cb, k4[1], nv[1], s0, U * 64, ts, q, qx, have_qx);
+        if (2 * U * 64 < nsl) offer_lists<U>(a, cb, k4[2], nv[2], s0, 2 * U * 64, ts, q, qx, have_qx);
+        if (3 * U * 64 < nsl) offer_lists<U>(a, cb, k4[3], nv[3], s0, 3 * U * 64, ts, q, qx, have_qx);
+        cb.compact();
+        emit_candidates(a, cb.buf, cb.n, base, xs, nullptr);
+        return;
+    }
+    {
         WaveTopM<2> ca;
         ca.init(bufA, M);
         for (int b = 0; b < nsl; b += 64 * U) {
@@ -467,8 +496,11 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
                 const int i = b + u * 64 + lane;
                 k[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
             }
-#pragma unroll
-            for (int u = 0; u < U; ++u) ca.push(k[u] != 0u, ((uint64_t)k[u] << 32) | (uint32_t)(b + u * 64 + lane));
+            static_assert(U == 4, "four pushes below");
+            ca.push(k[0] != 0u, ((uint64_t)k[0] << 32) | (uint32_t)(b + lane));
+            ca.push(k[1] != 0u, ((uint64_t)k[1] << 32) | (uint32_t)(b + 64 + lane));
+            ca.push(k[2] != 0u, ((uint64_t)k[2] << 32) | (uint32_t)(b + 128 + lane));
+            ca.push(k[3] != 0u, ((uint64_t)k[3] << 32) | (uint32_t)(b + 192 + lane));
         }
         ca.compact();
         if (ca.n >= M) ts = tk_score((uint32_t)(ca.buf[M - 1] >> 32));
@@ -482,9 +514,7 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
             k4[u] = i < nsl ? L[i] : make_uint4(0u, 0u, 0u, 0u);
             nv[u] = i < nsl ? (int)a.nvalid[s0 + i] : 0;
         }
-        int32_t row[U][4];
-        list_rows<U>(a, k4, nv, s0, b, row);
-        offer_lists<U>(a, cb, k4, nv, row, s0, b, ts, q, qx, have_qx);
+        offer_lists<U>(a, cb, k4, nv, s0, b, ts, q, qx, have_qx);
     }
     cb.compact();
     emit_candidates(a, cb.buf, cb.n, base, xs, nullptr);
