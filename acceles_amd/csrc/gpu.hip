@@ -97,6 +97,7 @@ struct acc_gpu_matrix {
     uint64_t *d_resc = nullptr;      // split * kMaxM
     int32_t *d_crow = nullptr, *d_cscore = nullptr;  // split * kMaxM (candidate outputs)
     int64_t *d_part_slice = nullptr;                 // split + 1
+    int64_t max_part_slices = 0;                     // slices of the largest partition
     BatchWs ws;                      // batched-path workspace (grown on demand)
     // batched path on the matrix cores (slices expanded from the packets in LDS)
     uint8_t *d_nvalid = nullptr;     // valid rows per slice
@@ -488,8 +489,12 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     if ((rc = check_launch()) != ACC_OK) return rc;
     // per wavefront: two key buffers, the int8 query (rescans), the FP32 query (rescoring)
     const size_t lds_sel = (size_t)ACC_WAVES * 256 * 8 + (size_t)ACC_WAVES * m->cols_pad * 5;
-    if ((rc = lds_limit((const void *)get_dense_select(), lds_sel)) != ACC_OK) return rc;
-    LAUNCH(m, 2, get_dense_select(), dim3(m->split, (nq + ACC_WAVES - 1) / ACC_WAVES), dim3(ACC_WG), lds_sel, st, a);
+    // partitions past the register path (more than 256 slices, or M > 32):
+    // the variant that also holds up to 1024 slices' lists in registers (more
+    // registers, fewer wavefronts per SIMD)
+    const dense_fn fsel = get_dense_select(m->max_part_slices > 256 || M > 32);
+    if ((rc = lds_limit((const void *)fsel, lds_sel)) != ACC_OK) return rc;
+    LAUNCH(m, 2, fsel, dim3(m->split, (nq + ACC_WAVES - 1) / ACC_WAVES), dim3(ACC_WG), lds_sel, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
     if (!out) return ACC_OK;
     const int total = m->split * M;  // <= kMaxCK = 4096 = 4 lists of 64*16
@@ -776,6 +781,9 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         TRY(dalloc(m, &m->d_cscore, (size_t)m->split * kMaxM));
         TRY(dalloc(m, &m->d_part_slice, (size_t)m->split + 1));
         COPY(m->d_part_slice, pk->part_slice_begin.data(), ((size_t)m->split + 1) * 8);
+        for (int p = 0; p < m->split; ++p)
+            m->max_part_slices = std::max<int64_t>(m->max_part_slices,
+                                                   pk->part_slice_begin[(size_t)p + 1] - pk->part_slice_begin[(size_t)p]);
         COPY(m->d_words, pk->words.data(), pk->words.size() * 4);
         ZERO(m->d_words + pk->words.size(), (size_t)kWordPad * 4);
         COPY(m->d_slices, sl.data(), sl.size() * sizeof(uint2));

@@ -298,9 +298,10 @@ __device__ __forceinline__ void emit_candidates(const DenseArgs &a, const uint64
 //     the lower-row tie rule), then their FP32 rescoring (out != null).
 // Partitions of at most 256 slices (M <= 32, at most 64 candidates) keep
 // their lists in registers: one prefix sum appends the candidates, a count
-// ranks them.  Up to 1024 slices the lists are still read once into
-// registers, and offered to a collector; anything larger streams them twice
-// (threshold, then offers).
+// ranks them.  In the WIDE variant, up to 1024 slices the lists are still
+// read once into registers and offered to a collector; anything else streams
+// them twice (threshold, then offers).
+template <bool WIDE>
 __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x >> 6, lane = lane_id();
@@ -451,7 +452,7 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
     cb.init(bufB, M);
     constexpr int U = 4;
     int32_t ts = INT32_MIN;
-    if (nsl <= 1024 && M <= 32) {
+    if (WIDE && nsl <= 1024 && M <= 32) {
         // every list in registers (16 per lane, loaded together, clamped into
         // the partition and masked after); the threshold is the M-th largest
         // full-slice maximum, then the lists are offered from the registers
@@ -486,7 +487,16 @@ __global__ __launch_bounds__(ACC_WG) void dense_select(DenseArgs a) {
         emit_candidates(a, cb.buf, cb.n, base, xs, nullptr);
         return;
     }
-    {
+    if (nsl <= 1024 && M <= 32) {  // the M-th largest maximum by extraction, maxima in registers
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int i = u * 64 + lane;
+            v[u] = i < nsl && a.nvalid[s0 + i] == 64 ? L[i].x : 0u;
+        }
+        const uint32_t t = wave_kth_u32<16>(v, M);
+        if (t != 0u) ts = tk_score(t);
+    } else {
         WaveTopM<2> ca;
         ca.init(bufA, M);
         for (int b = 0; b < nsl; b += 64 * U) {
@@ -544,7 +554,7 @@ __global__ __launch_bounds__(ACC_WG) void batch_topk_radix(DenseArgs a) {
 }
 
 dense_fn get_dense_prep() { return dense_prep; }
-dense_fn get_dense_select() { return dense_select; }
+dense_fn get_dense_select(bool wide) { return wide ? dense_select<true> : dense_select<false>; }
 dense_fn get_batch_topk_radix(int V) {
     switch (V) {
         case 1: return batch_topk_radix<1>;

@@ -14,8 +14,11 @@
 
 // y = A x with G lanes per row.  Lane g of a row sums its entries
 // g, g+G, g+2G, ... in that order with fmaf, then an xor butterfly of plain
-// adds.  Each wavefront owns a contiguous range of rows and takes B blocks of
-// 64/G rows per step.  The step's rows hold one contiguous range of entries:
+// adds.  A step is B blocks of 64/G rows; the steps are dealt to the
+// wavefronts in turn, so that the wavefronts in flight read neighbouring
+// entries (one contiguous range of rows per wavefront, i.e. thousands of
+// separate streams, measured 6% slower on a 20M-row matrix).  The step's rows
+// hold one contiguous range of entries:
 // for short rows (G <= 8) and when it fits kFp32Stage entries, it is first
 // copied into the wavefront's LDS area with 16-byte loads (whole cache lines,
 // each byte read once), one step ahead, and the G-lane order then reads LDS;
@@ -53,8 +56,10 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
     const int lane = lane_id(), g = lane & (G - 1);
     const int nblocks = (rows + RPI - 1) / RPI;
     const int gw = blockIdx.x * ACC_WAVES + (threadIdx.x >> 6), nw = gridDim.x * ACC_WAVES;
-    const int per = (nblocks + nw - 1) / nw;
-    const int b0 = gw * per, b1 = min(nblocks, b0 + per);
+    // steps of B blocks dealt to the wavefronts in turn (wavefront gw takes
+    // steps gw, gw + nw, ...): the wavefronts in flight read neighbouring
+    // entries, not one stream each
+    const int b0 = gw * B, b1 = nblocks, SB = nw * B;
     if (b0 >= b1) return;
     // row bounds are loaded unconditionally (row clamped into the matrix) and
     // masked after: a conditional load would be waited for before the step's
@@ -99,21 +104,21 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
     // ranges: this step's (rs, re), the next one's (ns, ne2); the one after is fetched in the loop
     uint32_t rs, re, ns = 0, ne2 = 0;
     range(b0, rs, re);
-    if (b0 + B < b1) range(b0 + B, ns, ne2);
+    if (b0 + SB < b1) range(b0 + SB, ns, ne2);
     bool have = issue(rs, re);  // this step's entries are in lv/lc
-    for (int b = b0; b < b1; b += B) {
+    for (int b = b0; b < b1; b += SB) {
         uint32_t nb[B], ne[B];
 #pragma unroll
         for (int t = 0; t < B; ++t) {  // the next step's row bounds
-            const int r = (b + B + t) * RPI + lane / G;
-            const bool ok = b + B + t < b1 && r < rows;
+            const int r = (b + SB + t) * RPI + lane / G;
+            const bool ok = b + SB + t < b1 && r < rows;
             const int rc = min(r, rows - 1);
             const uint32_t p0 = ptr[rc], p1 = ptr[rc + 1];
             nb[t] = ok ? p0 : 0u;
             ne[t] = ok ? p1 : 0u;
         }
         uint32_t ms = 0, me = 0;
-        if (b + 2 * B < b1) range(b + 2 * B, ms, me);
+        if (b + 2 * SB < b1) range(b + 2 * SB, ms, me);
         if (have) {
             const uint32_t sa = rs & ~3u, cnt = re - sa;
 #pragma unroll
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
             }
             wave_sync();
             // the next step's entries fly while this one is summed from LDS
-            const bool next = b + B < b1 && issue(ns, ne2);
+            const bool next = b + SB < b1 && issue(ns, ne2);
             // the first U entries per lane of all B blocks: their LDS reads issued together (clamped
             // into the staged range, masked after), then longer rows entry by entry
             const uint32_t last = cnt ? cnt - 1u : 0u;  // clamped reads stay in the staged range
@@ -210,7 +215,7 @@ __global__ __launch_bounds__(ACC_WG) void fp32_spmv(const uint32_t *ptr, const i
             jb[t] = nb[t];
             je[t] = ne[t];
         }
-        have = b + B < b1 && issue(ns, ne2);
+        have = b + SB < b1 && issue(ns, ne2);
         rs = ns;
         re = ne2;
         ns = ms;

@@ -683,7 +683,7 @@ batch_fn get_batch_select();                                   // k_batch.hip
 batch_fn get_batch_final(int R);
 typedef void (*dense_fn)(DenseArgs);
 dense_fn get_dense_prep();                                     // k_dense.hip
-dense_fn get_dense_select();
+dense_fn get_dense_select(bool wide);
 dense_fn get_batch_topk_radix(int V);
 dense_fn get_batch_topk(int R);
 dense_fn get_pk_scores_8(int E, int vm);                       // k_pkmfma_a.hip
