@@ -28,7 +28,7 @@ static constexpr int kMaxK = 960;    // Top-K limit of the FP32 path (collector 
 static constexpr int kMaxCK = 4096;  // candidates of the low-bit path
 static constexpr size_t kWordPad = 32 * 64;  // dwords past the packets (>= widest group, 22 x 64)
 static constexpr int kPkMaxCols = 1024;  // matrix-core scores: B operands of 32 column blocks in registers
-static constexpr int kPkMaxE = 18;       // entry widths instantiated for them
+static constexpr int kPkMaxE = 19;       // image-packet entry widths instantiated for them
 static constexpr size_t kPkLdsMax = 160 * 1024;  // LDS of a pk_scores workgroup at most
 
 // Per-batch device workspace of the batched path.  The handle owns one;
@@ -98,6 +98,10 @@ struct acc_gpu_matrix {
     int32_t *d_crow = nullptr, *d_cscore = nullptr;  // split * kMaxM (candidate outputs)
     int64_t *d_part_slice = nullptr;                 // split + 1
     int64_t max_part_slices = 0;                     // slices of the largest partition
+    uint32_t *d_pwords = nullptr;                    // image packets (matrix-core scores), E = 8 + pcb
+    uint2 *d_pslices = nullptr;
+    int pcb = 0;
+    int64_t pbytes = 0;                              // image-packet bytes read per pass (words + slice table)
     BatchWs ws;                      // batched-path workspace (grown on demand)
     // batched path on the matrix cores (slices expanded from the packets in LDS)
     uint8_t *d_nvalid = nullptr;     // valid rows per slice
@@ -384,6 +388,9 @@ DenseArgs dense_args(const acc_gpu_matrix *m) {
     std::memset(&a, 0, sizeof(a));
     a.words = m->d_words;
     a.slices = m->d_slices;
+    a.pwords = m->d_pwords;
+    a.pslices = m->d_pslices;
+    a.pcb = m->pcb;
     a.perm = m->d_perm;
     a.part_slice = m->d_part_slice;
     a.nvalid = m->d_nvalid;
@@ -417,19 +424,13 @@ int batch_layout(const acc_gpu_matrix *m, int nq) {
     return m->dense_ok && nq >= m->dense_min_nq ? ACC_BATCH_DENSE : ACC_BATCH_SPARSE;
 }
 
-// Value mode of the image bytes (k_pkmfma.hpp): 1 unsigned 8-bit (stored as
-// v - 128), 2 signed below 8 bits (sign-extended), 0 the field as is.
-int pk_value_mode(const acc_gpu_matrix *m) {
-    if (m->u8off) return 1;
-    return m->is_signed && m->bits < 8 ? 2 : 0;
-}
-
 // B registers for 8, 16 or 32 column blocks: the smallest that holds the
-// matrix (the multiplications run over all of them, zero past nkb).
+// matrix (the multiplications run over all of them, zero past nkb); E = the
+// image packets' entry bits.
 dense_fn pk_kernel(const acc_gpu_matrix *m) {
-    const int E = m->entry_bits, vm = pk_value_mode(m);
-    if (m->nkb > 16) return get_pk_scores_32(E, vm);
-    return m->nkb > 8 ? get_pk_scores_16(E, vm) : get_pk_scores_8(E, vm);
+    const int E = 8 + m->pcb;
+    if (m->nkb > 16) return get_pk_scores_32(E);
+    return m->nkb > 8 ? get_pk_scores_16(E) : get_pk_scores_8(E);
 }
 
 // Batched low-bit queries on the matrix cores: dense_prep, pk_scores (slices
@@ -816,10 +817,26 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
                 m->dense_why = "scores may reach 2^25 (too many columns for the value width)";
             else if (m->cols > kPkMaxCols)
                 m->dense_why = "more than 1024 columns";
-            else if (m->entry_bits > kPkMaxE)
-                m->dense_why = "entries wider than 18 bits";
+            else if (8 + std::max(2, acc::bits_for(m->cols + 1)) > kPkMaxE)
+                m->dense_why = "image-packet entries wider than 19 bits";
             else
                 m->dense_ok = true;
+        }
+        if (m->dense_ok) {
+            // the matrix-core scores' own copy of the slices: image bytes and
+            // cols - column per entry, repeated columns merged (k_pkmfma.hpp)
+            m->pcb = std::max(2, acc::bits_for(m->cols + 1));
+            std::vector<uint32_t> pw;
+            std::vector<std::pair<uint32_t, uint32_t>> ps;
+            TRY(acc::image_packets(pk, m->u8off, m->pcb, pw, ps));
+            std::vector<uint2> psl(ps.size());
+            for (size_t i = 0; i < ps.size(); ++i) psl[i] = make_uint2(ps[i].first, ps[i].second);
+            TRY(dalloc(m, &m->d_pwords, pw.size() + kWordPad));
+            TRY(dalloc(m, &m->d_pslices, psl.size()));
+            COPY(m->d_pwords, pw.data(), pw.size() * 4);
+            ZERO(m->d_pwords + pw.size(), (size_t)kWordPad * 4);
+            COPY(m->d_pslices, psl.data(), psl.size() * sizeof(uint2));
+            m->pbytes = (int64_t)pw.size() * 4 + (int64_t)psl.size() * 8;
         }
     }
     TRY(dalloc(m, &m->d_xstage, (size_t)m->cols));
@@ -853,7 +870,7 @@ extern "C" int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info
     info->device_bytes = m->device_bytes;
     info->batch_mode = m->batch_mode;
     info->dense_ok = m->dense_ok ? 1 : 0;
-    info->dense_bytes = 0;
+    info->dense_bytes = m->pbytes;
     return ACC_OK;
 }
 

@@ -76,6 +76,16 @@ int topk_lowbit_batch_ws(acc_gpu_matrix *m, BatchWorkspace *w, const float *x_de
 // acc_permute_rows applies (out row i = in row order[i]).
 std::vector<int32_t> row_shuffle(int32_t rows, uint64_t seed);
 
+// Image packets: the slices of pk re-encoded for the matrix-core scores.  Per
+// lane (row) a bit stream of (8 + cb)-bit entries byte | ((cols - column) << 8),
+// byte = the LDS image's value (v - 128 with u8off, else v, as a byte), a
+// row's repeated columns summed into one entry (the caller checked that the
+// sums fit a byte), then zero bits; lanes interleaved per dword as in pk.
+// slices[s] = (offset in units of 64 dwords, dwords per lane).  cb must hold
+// cols (2^cb > cols).  Returns ACC_OK or an error for out-of-range sums.
+int image_packets(const acc_packed *pk, int u8off, int cb, std::vector<uint32_t> &words,
+                  std::vector<std::pair<uint32_t, uint32_t>> &slices);
+
 // Device-memory admission: byte reservations against a fixed limit, each
 // checked and taken under one lock, so concurrent callers cannot both pass
 // against the same room.

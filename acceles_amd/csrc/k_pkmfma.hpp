@@ -77,31 +77,19 @@ __device__ __forceinline__ void lds_barrier() {
 // bits, and signed 8-bit whose field is the two's-complement byte), unsigned
 // 8-bit stored as v - 128 (the image background is -128 and 128 * sum(query)
 // is added back per query), and signed below 8 bits (sign-extended).
-enum { PK_PLAIN = 0, PK_U8 = 1, PK_SIGNED = 2 };
-
-// Entry j (0..31) of a packet group (E dwords of one lane = 32 entries of its
-// row) stored into the row's image line (row_base = row * stride): the byte
-// of column c at row_base + c.  A padding entry (all bits zero) goes to the
-// line's byte `cols_pad`, which no MFMA operand reads.  With DUP (a row may
-// name a column twice) the value is added to the byte instead (the host
-// checked that every such sum fits it); a padding entry then adds 0.
-template <int E, int VM, bool DUP>
-__device__ __forceinline__ void pk_entry(const uint32_t (&w)[E], int j, unsigned char *img, uint32_t row_base,
-                                         uint32_t cols_pad, int cb, int vb, uint32_t cmask) {
+// Entry j (0..31) of an image-packet group (E dwords of one lane = 32
+// entries of its row) stored into the row's image line: an entry is
+// byte | (c' << 8) with c' = cols - column, and its byte (already the image's
+// value) goes to rc - c' (rc = row * stride + cols).  An all-zero entry --
+// the padding after a row's last entry, or a group past the slice's end,
+// which loads as zeros -- writes 0 to column `cols`: no column of the matrix,
+// inside the row's line, and met only by zero B operands.
+template <int E>
+__device__ __forceinline__ void pk_entry(const uint32_t (&w)[E], int j, unsigned char *img, uint32_t rc, int cb) {
     const int pos = j * E, wi = pos >> 5, sh = pos & 31;
     const uint32_t hi = (wi + 1 < E) ? w[wi + 1] : 0u;
     const uint32_t bits = (sh + E <= 32) ? (w[wi] >> sh) : __builtin_amdgcn_alignbit(hi, w[wi], sh);
-    const uint32_t c = bits & cmask;
-    uint32_t byte;
-    if (VM == PK_SIGNED) byte = (uint32_t)__builtin_amdgcn_sbfe(bits, cb, vb);
-    else byte = __builtin_amdgcn_ubfe(bits, cb, vb);
-    if (DUP) {
-        unsigned char *p = img + row_base + c;
-        *p = (unsigned char)(*p + byte);
-    } else {
-        if (VM == PK_U8) byte ^= 0x80u;
-        img[row_base + (bits > cmask ? c : cols_pad)] = (unsigned char)byte;
-    }
+    img[rc - __builtin_amdgcn_ubfe(bits, 8, cb)] = (unsigned char)bits;
 }
 
 // Wavefront roles: up to kPkMfmaWaves MFMA wavefronts (one 32-query tile
@@ -129,10 +117,10 @@ struct PkCtx {
     unsigned char *smem;
     int lane, hi, wave;
     int64_t end;  // one past this workgroup's last slice
-    uint32_t cols_pad, stride, img_bytes, fill, cmask, row_base, a_base;
+    uint32_t cols_pad, stride, img_bytes, fill, rc, a_base;
     uint32_t nimg;  // slice images in LDS: 2 (fill, barrier, decode per round) or 3 (fill two rounds ahead)
-    int cb, vb;
-    __amdgpu_buffer_rsrc_t meta;  // a.slices
+    int cb;                       // column bits of the image packets
+    __amdgpu_buffer_rsrc_t meta;  // a.pslices
 };
 
 // Step kb of an MFMA round of parity P and the list work scheduled with it:
@@ -266,17 +254,17 @@ __device__ __forceinline__ void pk_mmain(const PkCtx &c, int qb, int tile, int64
     }
 }
 
-// Packet groups g = first, first + kPkDecodeWaves, ... of slice sl into image
-// p (DUPV: a row may name a column twice; then one wavefront decodes every
-// group, so that the additions to a byte are ordered).
-template <int E, int VM, bool DUPV>
+// Image-packet groups g = first, first + kPkDecodeWaves, ... of slice sl
+// into image p (a row names each column once, so the groups of a slice are
+// decoded in any order).
+template <int E>
 __device__ __forceinline__ void pk_decode_groups(const PkCtx &c, uint2 sl, int first, int step, unsigned char *p) {
-    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(c.a->words, sl);
+    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(c.a->pwords, sl);
     for (int g0 = first * E; g0 < (int)sl.y; g0 += step * E) {
         uint32_t w[E];
         load_group<E>(w, rs, g0);
 #pragma unroll
-        for (int j = 0; j < 32; ++j) pk_entry<E, VM, DUPV>(w, j, p, c.row_base, c.cols_pad, c.cb, c.vb, c.cmask);
+        for (int j = 0; j < 32; ++j) pk_entry<E>(w, j, p, c.rc, c.cb);
     }
 }
 
@@ -305,29 +293,25 @@ __device__ __forceinline__ uint2 pk_slice_meta(const PkCtx &c, int64_t s) {
 // two images, image nx is filled first (then a barrier); with three, image
 // `nf` (slice s + 2's) is filled for the next round and image nx was filled
 // in the previous one.  sl1, sl2 advance by one slice.
-template <int E, int VM, bool DEC, int P>
+template <int E, int P>
 __device__ __forceinline__ void pk_dround(const PkCtx &c, uint32_t (&w)[2][E], int dw, int64_t s, uint2 &sl1,
                                           uint2 &sl2, uint32_t nx, uint32_t nf) {
     const DenseArgs &a = *c.a;
     unsigned char *nxt = c.smem + nx * c.img_bytes;
     const int64_t s3 = s + 3;
     const uint2 s3m = pk_slice_meta(c, s3);
-    if constexpr (DEC) load_group<E>(w[1 - P], slice_rsrc(a.words, sl2), dw * E);
+    load_group<E>(w[1 - P], slice_rsrc(a.pwords, sl2), dw * E);
     if (c.nimg == 2) {
         pk_fill(c, nxt, c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
         lds_barrier();  // the image holds the background
     } else {
         pk_fill(c, c.smem + nf * c.img_bytes, c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
     }
-    if constexpr (DEC) {
-        if (dw * E < (int)sl1.y) {
+    if (dw * E < (int)sl1.y) {
 #pragma unroll
-            for (int j = 0; j < 32; ++j) pk_entry<E, VM, false>(w[P], j, nxt, c.row_base, c.cols_pad, c.cb, c.vb, c.cmask);
-            if ((dw + kPkDecodeWaves) * E < (int)sl1.y)
-                pk_decode_groups<E, VM, false>(c, sl1, dw + kPkDecodeWaves, kPkDecodeWaves, nxt);
-        }
-    } else if (dw == 0) {
-        pk_decode_groups<E, VM, true>(c, sl1, 0, 1, nxt);
+        for (int j = 0; j < 32; ++j) pk_entry<E>(w[P], j, nxt, c.rc, c.cb);
+        if ((dw + kPkDecodeWaves) * E < (int)sl1.y)
+            pk_decode_groups<E>(c, sl1, dw + kPkDecodeWaves, kPkDecodeWaves, nxt);
     }
     sl1 = sl2;
     sl2 = s3 < c.end ? s3m : make_uint2(0u, 0u);
@@ -337,7 +321,7 @@ __device__ __forceinline__ void pk_dround(const PkCtx &c, uint32_t (&w)[2][E], i
 // A decode wavefront (dw = 0 .. kPkDecodeWaves-1) over slices s .. end-1
 // (loads past a slice's end return zeros and fetch nothing; slices past the
 // matrix read as empty).
-template <int E, int VM, bool DEC>
+template <int E>
 __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
     const DenseArgs &a = *c.a;
     uint32_t w[2][E];
@@ -346,10 +330,9 @@ __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
     const uint2 s1m = pk_slice_meta(c, s + 1), s2m = pk_slice_meta(c, s + 2);
     uint2 sl1 = s + 1 < c.end ? s1m : make_uint2(0u, 0u);
     uint2 sl2 = s + 2 < c.end ? s2m : make_uint2(0u, 0u);
-    if constexpr (DEC) load_group<E>(w[0], slice_rsrc(a.words, sl1), dw * E);
+    load_group<E>(w[0], slice_rsrc(a.pwords, sl1), dw * E);
     lds_barrier();  // the images filled
-    if constexpr (DEC) pk_decode_groups<E, VM, false>(c, s0m, dw, kPkDecodeWaves, c.smem);
-    else if (dw == 0) pk_decode_groups<E, VM, true>(c, s0m, 0, 1, c.smem);
+    pk_decode_groups<E>(c, s0m, dw, kPkDecodeWaves, c.smem);
     lds_barrier();  // the first slice decoded
     // slice i of the range lies in image i % nimg: round i decodes slice i + 1
     // (image nx) and, with three images, fills slice i + 2's (image nf)
@@ -358,15 +341,15 @@ __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
         nx = nx + 1 == c.nimg ? 0u : nx + 1;
         nf = nf + 1 == c.nimg ? 0u : nf + 1;
     };
-    pk_dround<E, VM, DEC, 0>(c, w, dw, s, sl1, sl2, nx, nf);
+    pk_dround<E, 0>(c, w, dw, s, sl1, sl2, nx, nf);
     ++s;
     for (;;) {
         if (s >= c.end) return;
         advance();
-        pk_dround<E, VM, DEC, 1>(c, w, dw, s, sl1, sl2, nx, nf);
+        pk_dround<E, 1>(c, w, dw, s, sl1, sl2, nx, nf);
         if (++s >= c.end) return;
         advance();
-        pk_dround<E, VM, DEC, 0>(c, w, dw, s, sl1, sl2, nx, nf);
+        pk_dround<E, 0>(c, w, dw, s, sl1, sl2, nx, nf);
         ++s;
     }
 }
@@ -384,12 +367,12 @@ __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
 // per round (the decode wavefronts' fill, then the end of the round).
 //
 // Image of a slice (LDS): row r's bytes at r * stride + column, stride =
-// cols_pad + 16; the 16 bytes past cols_pad take the padding entries.  The A
+// cols_pad + 16; the bytes past cols meet zero B operands.  The A
 // operand (row tile rt, column block kb) of lane l is the 16 bytes of row
 // rt*32 + l%32 at column kb*32 + (l/32)*16: with stride/4 = 8m + 4, any 8
 // consecutive lanes read disjoint banks.  The query table pairs each B byte
 // with the same column.
-template <int E, int VM, int KB>
+template <int E, int KB>
 __global__ __launch_bounds__(PkShape::kThreads)
 __attribute__((amdgpu_waves_per_eu(PkShape::kWavesPerEU, PkShape::kWavesPerEU))) void pk_scores(DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -411,34 +394,27 @@ __attribute__((amdgpu_waves_per_eu(PkShape::kWavesPerEU, PkShape::kWavesPerEU)))
     c.stride = c.cols_pad + 16u;
     c.img_bytes = 64u * c.stride;
     c.nimg = (uint32_t)a.nimg;
-    c.fill = VM == PK_U8 ? 0x80808080u : 0u;
-    c.cmask = (1u << a.col_bits) - 1u;
-    c.row_base = (uint32_t)lane * c.stride;                            // scatter: lane = row
+    c.fill = a.u8off ? 0x80808080u : 0u;                               // the image byte of an absent entry
+    c.rc = (uint32_t)lane * c.stride + (uint32_t)a.cols;                 // scatter: lane = row
     c.a_base = (uint32_t)(lane & 31) * c.stride + (uint32_t)hi * 16u;  // A operand of row tile 0
-    c.cb = a.col_bits;
-    c.vb = a.val_bits;
-    c.meta = __builtin_amdgcn_make_buffer_rsrc((void *)a.slices, (short)0, (int)(a.nslices * 8),
+    c.cb = a.pcb;
+    c.meta = __builtin_amdgcn_make_buffer_rsrc((void *)a.pslices, (short)0, (int)(a.nslices * 8),
                                                0x00020000 /* 32-bit data format */);
     if (wave < a.NT) pk_mmain<KB>(c, qb, wave, s0);
-    else if (a.dup) pk_dmain<E, VM, false>(c, wave - a.NT, s0);
-    else pk_dmain<E, VM, true>(c, wave - a.NT, s0);
+    else pk_dmain<E>(c, wave - a.NT, s0);
 }
 
-#define PK_CASES_VM(VM, KB)                    \
-    switch (E) {                               \
-        case 10: return pk_scores<10, VM, KB>; \
-        case 11: return pk_scores<11, VM, KB>; \
-        case 12: return pk_scores<12, VM, KB>; \
-        case 13: return pk_scores<13, VM, KB>; \
-        case 14: return pk_scores<14, VM, KB>; \
-        case 15: return pk_scores<15, VM, KB>; \
-        case 16: return pk_scores<16, VM, KB>; \
-        case 17: return pk_scores<17, VM, KB>; \
-        case 18: return pk_scores<18, VM, KB>; \
-        default: return nullptr;               \
+#define PK_CASES(KB)                       \
+    switch (E) {                           \
+        case 10: return pk_scores<10, KB>; \
+        case 11: return pk_scores<11, KB>; \
+        case 12: return pk_scores<12, KB>; \
+        case 13: return pk_scores<13, KB>; \
+        case 14: return pk_scores<14, KB>; \
+        case 15: return pk_scores<15, KB>; \
+        case 16: return pk_scores<16, KB>; \
+        case 17: return pk_scores<17, KB>; \
+        case 18: return pk_scores<18, KB>; \
+        case 19: return pk_scores<19, KB>; \
+        default: return nullptr;           \
     }
-
-#define PK_CASES(KB)                                    \
-    if (vm == PK_U8) { PK_CASES_VM(PK_U8, KB) }         \
-    if (vm == PK_SIGNED) { PK_CASES_VM(PK_SIGNED, KB) } \
-    PK_CASES_VM(PK_PLAIN, KB)

@@ -597,8 +597,10 @@ struct QTable {
 // ---------------------------------------------------------------------------
 //   dense_prep     quantizes the batch's queries into the B-operand table;
 //   pk_scores      v_mfma_i32_32x32x32_i8 over each 64-row slice and a block
-//                  of queries: decode wavefronts expand the slice's packets
-//                  into an LDS image of the A operands, each MFMA wavefront
+//                  of queries: decode wavefronts expand the slice's image
+//                  packets (the same slices re-encoded at upload: per entry
+//                  the image byte and cols - column, a row's repeated columns
+//                  merged) into an LDS image of the A operands, each MFMA wavefront
 //                  holds its query tile's B operands in registers; per (query,
 //                  slice) only the four best (score, row-in-slice) keys are
 //                  kept;
@@ -609,13 +611,13 @@ struct QTable {
 //   batch_topk     per query: its Top-K[q] of the rescored keys.
 //
 // Image of a slice (LDS): row r of the slice at r * (cols_pad + 16), one
-// byte per column; the 16 bytes past cols_pad take the padding entries.  The
+// byte per column; the padding entries write column cols.  The
 // A operand (row tile rt, column block kb) of lane l is the 16 bytes of row
 // rt*32 + l%32 at column kb*32 + (l/32)*16.  The query table uses the same
 // lane mapping for the B operand (query l%32 of a 32-query tile), so every k
 // is paired identically in both.  Unsigned 8-bit values are stored as v-128
 // (absent entries as -128) and 128 * sum(query) is added back per query;
-// other widths are stored as is.  Two images, then kPkLdsTail bytes that the
+// other widths are stored as is.  Two or three images, then kPkLdsTail bytes that the
 // chain's reads of column blocks past nkb (multiplied by zero B operands)
 // may reach.
 static constexpr int kPkLdsTail = 34 * 32;
@@ -627,6 +629,9 @@ static constexpr int kPkDecodeWaves = 4;
 struct DenseArgs {
     const uint32_t *words;       // sliced packets
     const uint2 *slices;
+    const uint32_t *pwords;      // image packets (pk_scores): (byte, cols - column) entries of pcb + 8 bits
+    const uint2 *pslices;
+    int pcb;
     const int32_t *perm;
     const int64_t *part_slice;   // split + 1
     const uint8_t *nvalid;       // valid rows per slice (a prefix of its lanes)
@@ -686,6 +691,6 @@ dense_fn get_dense_prep();                                     // k_dense.hip
 dense_fn get_dense_select(bool wide);
 dense_fn get_batch_topk_radix(int V);
 dense_fn get_batch_topk(int R);
-dense_fn get_pk_scores_8(int E, int vm);                       // k_pkmfma_a.hip
-dense_fn get_pk_scores_16(int E, int vm);                      // k_pkmfma_b.hip
-dense_fn get_pk_scores_32(int E, int vm);                      // k_pkmfma_c.hip
+dense_fn get_pk_scores_8(int E);                               // k_pkmfma_a.hip
+dense_fn get_pk_scores_16(int E);                              // k_pkmfma_b.hip
+dense_fn get_pk_scores_32(int E);                              // k_pkmfma_c.hip

@@ -210,6 +210,103 @@ extern "C" int acc_packed_info(const acc_packed *pk, acc_packed_info_t *info) {
     return ACC_OK;
 }
 
+namespace {
+
+// The kept entries of lane l of slice s: (column, signed value), in stream order.
+void lane_entries(const acc_packed *pk, int64_t s, int l, std::vector<std::pair<int32_t, int32_t>> &out) {
+    const int E = pk->entry_bits, cb = pk->col_bits, vb = pk->bits;
+    const uint32_t cmask = (1u << cb) - 1u, vmask = (1u << vb) - 1u;
+    const uint32_t *base = pk->words.data() + (size_t)pk->slice_off[(size_t)s] * 64;
+    const int64_t ndw = pk->slice_ndw[(size_t)s], nent = ndw * 32 / E;
+    out.clear();
+    for (int64_t j = 0; j < nent; ++j) {
+        const int64_t pos = j * E, w = pos >> 5;
+        const int sh = (int)(pos & 31);
+        const uint64_t lo = base[(size_t)w * 64 + l];
+        const uint64_t hi = w + 1 < ndw ? base[(size_t)(w + 1) * 64 + l] : 0;
+        const uint32_t x = (uint32_t)(((hi << 32) | lo) >> sh);
+        const uint32_t v = (x >> cb) & vmask;
+        if (v == 0) break;
+        int32_t sv = (int32_t)v;
+        if (pk->is_signed && (v >> (vb - 1))) sv = (int32_t)(v | ~vmask);
+        out.push_back({(int32_t)(x & cmask), sv});
+    }
+}
+
+// The image-packet entries of one lane: repeated columns summed (column order).
+bool lane_image_entries(const acc_packed *pk, int64_t s, int l, int u8off, std::vector<std::pair<int32_t, int32_t>> &ent,
+                        std::vector<uint32_t> &out) {
+    lane_entries(pk, s, l, ent);
+    std::stable_sort(ent.begin(), ent.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    out.clear();
+    for (size_t i = 0; i < ent.size();) {
+        size_t j = i;
+        int64_t sum = 0;
+        while (j < ent.size() && ent[j].first == ent[i].first) sum += ent[j++].second;
+        const int64_t b = u8off ? sum - 128 : sum;
+        if (b < -128 || b > 127) return false;
+        out.push_back((uint32_t)(b & 0xFF) | ((uint32_t)(pk->cols - ent[i].first) << 8));
+        i = j;
+    }
+    return true;
+}
+
+}  // namespace
+
+int acc::image_packets(const acc_packed *pk, int u8off, int cb, std::vector<uint32_t> &words,
+                       std::vector<std::pair<uint32_t, uint32_t>> &slices) {
+    const int64_t NS = pk->nslices();
+    const int E = cb + 8;
+    if (cb < 1 || E > 32 || (int64_t(1) << cb) <= pk->cols) return fail(ACC_ERR_ARG, "image packets: column bits");
+    slices.assign((size_t)NS, {0u, 0u});
+    std::vector<int64_t> cnt((size_t)NS, 0);
+    int bad = 0;
+#pragma omp parallel
+    {
+        std::vector<std::pair<int32_t, int32_t>> ent;
+        std::vector<uint32_t> out;
+#pragma omp for schedule(dynamic, 64) reduction(| : bad)
+        for (int64_t s = 0; s < NS; ++s) {
+            int64_t mx = 0;
+            for (int l = 0; l < 64; ++l) {
+                if (pk->perm[(size_t)s * 64 + l] < 0) continue;
+                if (!lane_image_entries(pk, s, l, u8off, ent, out)) bad = 1;
+                mx = std::max<int64_t>(mx, (int64_t)out.size());
+            }
+            cnt[(size_t)s] = mx;
+        }
+    }
+    if (bad) return fail(ACC_ERR_RANGE, "image packets: a row's values in one column sum past one byte");
+    uint64_t off = 0;
+    for (int64_t s = 0; s < NS; ++s) {
+        const uint32_t ndw = (uint32_t)((cnt[(size_t)s] * E + 31) / 32);
+        if (off > 0xFFFFFFFFull) return fail(ACC_ERR_RANGE, "image packets exceed 2^38 bytes");
+        slices[(size_t)s] = {(uint32_t)off, ndw};
+        off += ndw;
+    }
+    words.assign((size_t)off * 64, 0u);
+#pragma omp parallel
+    {
+        std::vector<std::pair<int32_t, int32_t>> ent;
+        std::vector<uint32_t> out;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t s = 0; s < NS; ++s) {
+            uint32_t *base = words.data() + (size_t)slices[(size_t)s].first * 64;
+            for (int l = 0; l < 64; ++l) {
+                if (pk->perm[(size_t)s * 64 + l] < 0) continue;
+                lane_image_entries(pk, s, l, u8off, ent, out);
+                for (size_t j = 0; j < out.size(); ++j) {
+                    const int64_t pos = (int64_t)j * E, w = pos >> 5;
+                    const int sh = (int)(pos & 31);
+                    base[(size_t)w * 64 + l] |= out[j] << sh;
+                    if (sh + E > 32) base[(size_t)(w + 1) * 64 + l] |= out[j] >> (32 - sh);
+                }
+            }
+        }
+    }
+    return ACC_OK;
+}
+
 extern "C" int acc_unpack_lowbit(const acc_packed *pk, int64_t *ptr, int32_t *idx, int32_t *qval) {
     if (!pk || !ptr) return fail(ACC_ERR_ARG, "null argument");
     const int E = pk->entry_bits, cb = pk->col_bits, vb = pk->bits;

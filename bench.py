@@ -696,7 +696,7 @@ def main():
         names = ["prep", "scores", "select_rescore", "final"]
         t_sc = bt_ms[1]
         cols_pad = (a.cols + 31) // 32 * 32
-        bs_bytes = pk.info.stream_bytes + B * a.cols * 4
+        bs_bytes = g.info.dense_bytes + B * a.cols * 4  # the image packets pk_scores streams, and the queries
         bs_ops = 2.0 * nslots * cols_pad * ((B + 31) // 32 * 32)
         sc_kernel = "pk_scores"
     else:
@@ -808,8 +808,8 @@ def main():
         "lowbit_batched_vs_fp32_speedup": round(qps1 / fqps, 3),  # both one at a time
         "batched_equals_single": same,
         "batch_launch_us": dict(zip(names, [round(float(v) * 1e3, 2) for v in bt_ms])),
-        "batch_scoring": "matrix cores, slices expanded from the packets in LDS" if dense else "sliced packets (dot4)",
-        "dense_bytes": g.info.dense_bytes,
+        "batch_scoring": "matrix cores, slices expanded from the image packets in LDS" if dense else "sliced packets (dot4)",
+        "image_packet_bytes": g.info.dense_bytes,
         "batched_other_layout_qps": None if other_qps is None else round(other_qps, 2),
         "lowbit_launch_us": [round(float(v) * 1e3, 2) for v in lb_ms],
         "fp32_launch_us": [round(float(v) * 1e3, 2) for v in fp_ms],

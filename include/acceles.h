@@ -261,7 +261,8 @@ typedef struct {
     int64_t device_bytes;  /* total device memory held by the handle         */
     int32_t batch_mode;    /* ACC_BATCH_* in effect                          */
     int32_t dense_ok;      /* ACC_BATCH_DENSE can score the matrix exactly   */
-    int64_t dense_bytes;   /* device bytes held for it beyond the packets: 0 */
+    int64_t dense_bytes;   /* its image packets (the packets re-encoded for the
+                              matrix cores), read per pass; 0 if !dense_ok   */
 } acc_gpu_info_t;
 
 int acc_gpu_matrix_info(const acc_gpu_matrix *m, acc_gpu_info_t *info);
