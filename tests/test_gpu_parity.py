@@ -250,6 +250,9 @@ def test_lowbit_batch_uneven_k(acc, gpu, mode):
     (40000, 4096, (0, 300), 4, 16, False, 24),   # 4096 columns: query columns past what the kernels hold in registers
     (50000, 200, (10, 30), 8, 2, False, 20),     # partitions of 391 slices (threshold by extraction)
     (150000, 200, (10, 30), 8, 2, False, 20),    # partitions of 1172 slices (threshold by collectors)
+    (20000, 512, (40, 100), 6, 16, True, 40),    # signed 6-bit, 512 columns: image entries of 8 + 10 bits
+    (20000, 256, (10, 60), 8, 8, False, 64),     # 256 columns: image entries one column bit wider than the packets'
+    (5000, 3, (1, 3), 8, 4, False, 20),          # three columns: the narrowest image entries (8 + 2 bits)
 ])
 def test_lowbit_batch_matches_oracle(acc, gpu, mode, rows, cols, deg, bits, split, signed, nq):
     kind = 1 if isinstance(deg[0], float) else 0
