@@ -268,11 +268,20 @@ __device__ __forceinline__ void pk_decode_groups(const PkCtx &c, uint2 sl, int f
     }
 }
 
-// The background into image p, by the decode wavefronts' lanes dl.
+// The background into image p, by the decode wavefronts' lanes dl (four
+// stores per loop test, at constant offsets).
 __device__ __forceinline__ void pk_fill(const PkCtx &c, unsigned char *p, uint32_t n16, uint32_t dl) {
+    constexpr uint32_t T = kPkDecodeWaves * 64;
     pv4i *p4 = (pv4i *)p;
     const pv4i f = {(int)c.fill, (int)c.fill, (int)c.fill, (int)c.fill};
-    for (uint32_t i = dl; i < n16; i += kPkDecodeWaves * 64) p4[i] = f;
+    uint32_t i = dl;
+    for (; i + 3 * T < n16; i += 4 * T) {
+        p4[i] = f;
+        p4[i + T] = f;
+        p4[i + 2 * T] = f;
+        p4[i + 3 * T] = f;
+    }
+    for (; i < n16; i += T) p4[i] = f;
 }
 
 // Metadata of slice min(s, end - 1) as a vector load: a scalar load would be
