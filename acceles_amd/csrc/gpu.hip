@@ -109,7 +109,6 @@ struct acc_gpu_matrix {
     std::string dense_why;           // why not, otherwise
     int batch_mode = ACC_BATCH_AUTO;
     int dense_min_nq = 16;           // ACC_BATCH_AUTO, untuned: matrix cores from this batch size on
-    bool dup = false;                // some row names a column twice
     // ACC_BATCH_AUTO after acc_gpu_autotune: the layout measured fastest at
     // tune_nq[i], used for batches of tune_nq[i] .. tune_nq[i+1]-1 queries
     int ntune = 0;
@@ -402,7 +401,6 @@ DenseArgs dense_args(const acc_gpu_matrix *m) {
     a.val_bits = m->bits;
     a.is_signed = m->is_signed;
     a.u8off = m->u8off;
-    a.dup = m->dup ? 1 : 0;
     a.nimg = 2;
     a.split = m->split;
     a.ptr = m->d_ptr;
@@ -804,7 +802,6 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
         m->cols_pad = (m->cols + 31) & ~31;
         m->nkb = m->cols_pad / 32;
         m->u8off = (!m->is_signed && m->bits == 8) ? 1 : 0;
-        m->dup = pk->dup_rows > 0;
         {
             // a byte of the slice image holds the sum of a row's values in one
             // column (rows may name a column twice): v - 128 for unsigned

@@ -372,8 +372,9 @@ __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
 // a query's four-key lists of one workgroup fill whole cache lines in L2
 // before they are written back.  KB: column blocks the B registers hold (nkb <= KB; the chain runs over
 // all KB with zero B operands past nkb, so that it has no control flow).
-// Every wavefront passes the same barriers: two before the first round, two
-// per round (the decode wavefronts' fill, then the end of the round).
+// Every wavefront passes the same barriers: two before the first round, then
+// per round one (three images: the end of the round) or two (two images: the
+// decode wavefronts' fill, then the end of the round).
 //
 // Image of a slice (LDS): row r's bytes at r * stride + column, stride =
 // cols_pad + 16; the bytes past cols meet zero B operands.  The A

@@ -636,7 +636,7 @@ struct DenseArgs {
     const int64_t *part_slice;   // split + 1
     const uint8_t *nvalid;       // valid rows per slice (a prefix of its lanes)
     int64_t nslices;
-    int cols, cols_pad, nkb, col_bits, val_bits, is_signed, u8off, dup;
+    int cols, cols_pad, nkb, col_bits, val_bits, is_signed, u8off;
     int nimg;                    // pk_scores: slice images in LDS (2 or 3)
     const float *X;              // nq x cols FP32 queries
     int nq, NT, nqb;             // queries; 32-query tiles per block; blocks
