@@ -422,12 +422,14 @@ int batch_layout(const acc_gpu_matrix *m, int nq) {
     return m->dense_ok && nq >= m->dense_min_nq ? ACC_BATCH_DENSE : ACC_BATCH_SPARSE;
 }
 
-// B registers for 8, 16 or 32 column blocks: the smallest that holds the
-// matrix (the multiplications run over all of them, zero past nkb); E = the
-// image packets' entry bits.
+// B registers for 8, 16, 24, 30 or 32 column blocks: the smallest that holds
+// the matrix (the multiplications run over all of them, zero past nkb); E =
+// the image packets' entry bits.
 dense_fn pk_kernel(const acc_gpu_matrix *m) {
     const int E = 8 + m->pcb;
-    if (m->nkb > 16) return get_pk_scores_32(E);
+    if (m->nkb > 30) return get_pk_scores_32(E);
+    if (m->nkb > 24) return get_pk_scores_30(E);
+    if (m->nkb > 16) return get_pk_scores_24(E);
     return m->nkb > 8 ? get_pk_scores_16(E) : get_pk_scores_8(E);
 }
 

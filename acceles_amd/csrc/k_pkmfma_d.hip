@@ -1,0 +1,5 @@
+// Instantiations of pk_scores with B registers for up to 24 column blocks
+// (one translation unit each, for build parallelism).
+#include "k_pkmfma.hpp"
+
+dense_fn get_pk_scores_24(int E) { PK_CASES(24) }

@@ -694,3 +694,5 @@ dense_fn get_batch_topk(int R);
 dense_fn get_pk_scores_8(int E);                               // k_pkmfma_a.hip
 dense_fn get_pk_scores_16(int E);                              // k_pkmfma_b.hip
 dense_fn get_pk_scores_32(int E);                              // k_pkmfma_c.hip
+dense_fn get_pk_scores_24(int E);                              // k_pkmfma_d.hip
+dense_fn get_pk_scores_30(int E);                              // k_pkmfma_e.hip

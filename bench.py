@@ -161,8 +161,8 @@ def precision_at(got, exact):
 # the sources each profiled kernel is built from: a counter file counts only while they are unchanged
 KERNEL_SOURCES = {
     "pk_scores": ["acceles_amd/csrc/k_pkmfma.hpp", "acceles_amd/csrc/k_pkmfma_a.hip", "acceles_amd/csrc/k_pkmfma_b.hip",
-                  "acceles_amd/csrc/k_pkmfma_c.hip", "acceles_amd/csrc/kernels_common.hpp",
-                  "acceles_amd/csrc/device_common.hpp"],
+                  "acceles_amd/csrc/k_pkmfma_c.hip", "acceles_amd/csrc/k_pkmfma_d.hip", "acceles_amd/csrc/k_pkmfma_e.hip",
+                  "acceles_amd/csrc/kernels_common.hpp", "acceles_amd/csrc/device_common.hpp"],
     "lowbit_slicemax": ["acceles_amd/csrc/k_lowbit.hip", "acceles_amd/csrc/kernels_common.hpp",
                         "acceles_amd/csrc/device_common.hpp"],
 }

@@ -253,6 +253,7 @@ def test_lowbit_batch_uneven_k(acc, gpu, mode):
     (20000, 512, (40, 100), 6, 16, True, 40),    # signed 6-bit, 512 columns: image entries of 8 + 10 bits
     (20000, 256, (10, 60), 8, 8, False, 64),     # 256 columns: image entries one column bit wider than the packets'
     (5000, 3, (1, 3), 8, 4, False, 20),          # three columns: the narrowest image entries (8 + 2 bits)
+    (20000, 700, (30, 90), 8, 16, False, 50),    # 22 column blocks: the 24-block chain
 ])
 def test_lowbit_batch_matches_oracle(acc, gpu, mode, rows, cols, deg, bits, split, signed, nq):
     kind = 1 if isinstance(deg[0], float) else 0
