@@ -401,7 +401,6 @@ DenseArgs dense_args(const acc_gpu_matrix *m) {
     a.val_bits = m->bits;
     a.is_signed = m->is_signed;
     a.u8off = m->u8off;
-    a.nimg = 2;
     a.split = m->split;
     a.ptr = m->d_ptr;
     a.idx = m->d_idx;
@@ -474,10 +473,9 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     if ((rc = check_launch()) != ACC_OK) return rc;
     dense_fn fs = pk_kernel(m);
     if (!fs) return fail(ACC_ERR_ARG, "unsupported entry width");
-    // three slice images when they fit (the decode runs a round ahead, one barrier per slice), else two
+    // two slice images (one multiplied while the other is cleared and refilled)
     const size_t img = (size_t)64 * (m->cols_pad + 16);
-    a.nimg = 3 * img + kPkLdsTail <= kPkLdsMax ? 3 : 2;
-    const size_t lds_s = (size_t)a.nimg * img + kPkLdsTail;
+    const size_t lds_s = 2 * img + kPkLdsTail;
     if ((rc = lds_limit((const void *)fs, lds_s)) != ACC_OK) return rc;
     // workgroups per query block: as many as fit the chip at once, a multiple
     // of 8 (the kernel keeps a slice range's query blocks on one XCD)

@@ -118,7 +118,7 @@ struct PkCtx {
     int lane, hi, wave;
     int64_t end;  // one past this workgroup's last slice
     uint32_t cols_pad, stride, img_bytes, fill, rc, a_base;
-    uint32_t nimg;  // slice images in LDS: 2 (fill, barrier, decode per round) or 3 (fill two rounds ahead)
+    uint32_t drow;  // decode wavefronts: the lane's row (16 * dw + lane % 16)
     int cb;                       // column bits of the image packets
     __amdgpu_buffer_rsrc_t meta;  // a.pslices
 };
@@ -130,7 +130,6 @@ template <int KB, int P, bool LISTS, int kb>
 __device__ __forceinline__ void pk_mstep(const PkCtx &c, PkMRegs &r, const pv4i (&B)[KB], const pv4i *A0,
                                          const pv4i *A1) {
     constexpr int RING = 4, DIST = 2;  // A operands: ring slots, column blocks ahead
-    constexpr int ZS = KB / 4;         // the decode wavefronts' fill barrier
     if constexpr (kb + DIST < KB) {    // constant offsets; blocks past nkb meet zero B operands
         r.ar[(kb + DIST) % RING][0] = A0[(kb + DIST) * 2];
         r.ar[(kb + DIST) % RING][1] = A1[(kb + DIST) * 2];
@@ -138,9 +137,6 @@ __device__ __forceinline__ void pk_mstep(const PkCtx &c, PkMRegs &r, const pv4i 
     __builtin_amdgcn_sched_barrier(0);
     r.acc[P][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(r.ar[kb % RING][0], B[kb], r.acc[P][0], 0, 0, 0);
     r.acc[P][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(r.ar[kb % RING][1], B[kb], r.acc[P][1], 0, 0, 0);
-    if constexpr (kb == ZS) {
-        if (c.nimg == 2) lds_barrier();  // two images: the next one's fill is complete
-    }
     // the previous slice's accumulators into the list, value v at step v*KB/32
 #pragma unroll
     for (int v = 0; v < 32; ++v)
@@ -230,9 +226,9 @@ __device__ __forceinline__ void pk_mmain(const PkCtx &c, int qb, int tile, int64
     PkMRegs r;
     lds_barrier();  // the images filled
     lds_barrier();  // the first slice decoded
-    // slice i of the range lies in image i % nimg
+    // slice i of the range lies in image i % 2
     uint32_t img = 0;
-    auto next_img = [&]() { img = img + 1 == c.nimg ? 0u : img + 1; };
+    auto next_img = [&]() { img ^= 1u; };
     pk_mround<KB, 0, false>(c, r, B, Dk, q, -1, img);
     int64_t s_prev = s;
     ++s;
@@ -254,34 +250,55 @@ __device__ __forceinline__ void pk_mmain(const PkCtx &c, int qb, int tile, int64
     }
 }
 
-// Image-packet groups g = first, first + kPkDecodeWaves, ... of slice sl
-// into image p (a row names each column once, so the groups of a slice are
-// decoded in any order).
+// Decode wavefront dw owns rows 16*dw .. 16*dw + 15 of every image: lane l
+// decodes row 16*dw + l%16, packet groups l/16, l/16 + 4, ... of that row's
+// stream (E dwords per group; a group past the slice's end loads as zeros).
+// Every byte of those rows is written by this wavefront only, in program
+// order, so clearing an image and expanding the next slice into it need no
+// barrier between them.
 template <int E>
-__device__ __forceinline__ void pk_decode_groups(const PkCtx &c, uint2 sl, int first, int step, unsigned char *p) {
-    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(c.a->pwords, sl);
-    for (int g0 = first * E; g0 < (int)sl.y; g0 += step * E) {
-        uint32_t w[E];
-        load_group<E>(w, rs, g0);
+struct PkDRegs {
+    uint32_t w[2][E];       // the first group set of the next slice, by round parity
+};
+
+// The E dwords of group set k of the lane's row (lane / 16 + 4k) of a slice.
+template <int E>
+__device__ __forceinline__ void pk_load_rows(uint32_t (&w)[E], __amdgpu_buffer_rsrc_t rs, const PkCtx &c, int k) {
+    const int g = (c.lane >> 4) + 4 * k;
 #pragma unroll
-        for (int j = 0; j < 32; ++j) pk_entry<E>(w, j, p, c.rc, c.cb);
-    }
+    for (int i = 0; i < E; ++i)
+        w[i] = __builtin_amdgcn_raw_buffer_load_b32(rs, ((g * E + i) * 64 + (int)c.drow) * 4, 0, 2 /* nt */);
 }
 
-// The background into image p, by the decode wavefronts' lanes dl (four
-// stores per loop test, at constant offsets).
-__device__ __forceinline__ void pk_fill(const PkCtx &c, unsigned char *p, uint32_t n16, uint32_t dl) {
-    constexpr uint32_t T = kPkDecodeWaves * 64;
-    pv4i *p4 = (pv4i *)p;
+// This wavefront's rows of image img back to the background: 16 rows =
+// stride 16-byte chunks (fewer LDS cycles than clearing the bytes the last
+// slice wrote, one at a time, at every width up to 1024 columns).
+__device__ __forceinline__ void pk_clear(const PkCtx &c, unsigned char *img, int dw) {
+    pv4i *p4 = (pv4i *)(img + (uint32_t)dw * 16u * c.stride);
     const pv4i f = {(int)c.fill, (int)c.fill, (int)c.fill, (int)c.fill};
-    uint32_t i = dl;
-    for (; i + 3 * T < n16; i += 4 * T) {
+    uint32_t i = (uint32_t)c.lane;
+    for (; i + 192u < c.stride; i += 256u) {
         p4[i] = f;
-        p4[i + T] = f;
-        p4[i + 2 * T] = f;
-        p4[i + 3 * T] = f;
+        p4[i + 64u] = f;
+        p4[i + 128u] = f;
+        p4[i + 192u] = f;
     }
-    for (; i < n16; i += T) p4[i] = f;
+    for (; i < c.stride; i += 64u) p4[i] = f;
+}
+
+// Slice sl (first group set in w, further sets loaded here) into image img.
+template <int E>
+__device__ __forceinline__ void pk_expand(const PkCtx &c, const uint32_t (&w)[E], uint2 sl, unsigned char *img) {
+#pragma unroll
+    for (int j = 0; j < 32; ++j) pk_entry<E>(w, j, img, c.rc, c.cb);
+    if (4 * E >= (int)sl.y) return;
+    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(c.a->pwords, sl);
+    for (int k = 1; 4 * k * E < (int)sl.y; ++k) {
+        uint32_t x[E];
+        pk_load_rows<E>(x, rs, c, k);
+#pragma unroll
+        for (int j = 0; j < 32; ++j) pk_entry<E>(x, j, img, c.rc, c.cb);
+    }
 }
 
 // Metadata of slice min(s, end - 1) as a vector load: a scalar load would be
@@ -295,33 +312,20 @@ __device__ __forceinline__ uint2 pk_slice_meta(const PkCtx &c, int64_t s) {
     return m;
 }
 
-// Decode round of parity P (the MFMA wavefronts multiply slice s): slice
-// s + 1 (metadata sl1) decoded into image `nx` -- group dw from the registers
-// w[P], requested a round earlier, and groups dw + kPkDecodeWaves, ...
-// loaded here; group dw of slice s + 2 (sl2) requested into w[1-P].  With
-// two images, image nx is filled first (then a barrier); with three, image
-// `nf` (slice s + 2's) is filled for the next round and image nx was filled
-// in the previous one.  sl1, sl2 advance by one slice.
+// Decode round of parity P (the MFMA wavefronts multiply slice s from image
+// P): slice s + 1 (metadata sl1, first group set in r.w[P], requested a round
+// earlier) expanded into image 1 - P after this wavefront's rows of it are
+// cleared; the first group set of slice s + 2 (sl2) requested into r.w[1-P].
+// sl1, sl2 advance by one slice.
 template <int E, int P>
-__device__ __forceinline__ void pk_dround(const PkCtx &c, uint32_t (&w)[2][E], int dw, int64_t s, uint2 &sl1,
-                                          uint2 &sl2, uint32_t nx, uint32_t nf) {
+__device__ __forceinline__ void pk_dround(const PkCtx &c, PkDRegs<E> &r, int dw, int64_t s, uint2 &sl1, uint2 &sl2) {
     const DenseArgs &a = *c.a;
-    unsigned char *nxt = c.smem + nx * c.img_bytes;
+    unsigned char *nxt = c.smem + (1 - P) * c.img_bytes;
     const int64_t s3 = s + 3;
     const uint2 s3m = pk_slice_meta(c, s3);
-    load_group<E>(w[1 - P], slice_rsrc(a.pwords, sl2), dw * E);
-    if (c.nimg == 2) {
-        pk_fill(c, nxt, c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
-        lds_barrier();  // the image holds the background
-    } else {
-        pk_fill(c, c.smem + nf * c.img_bytes, c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
-    }
-    if (dw * E < (int)sl1.y) {
-#pragma unroll
-        for (int j = 0; j < 32; ++j) pk_entry<E>(w[P], j, nxt, c.rc, c.cb);
-        if ((dw + kPkDecodeWaves) * E < (int)sl1.y)
-            pk_decode_groups<E>(c, sl1, dw + kPkDecodeWaves, kPkDecodeWaves, nxt);
-    }
+    pk_load_rows<E>(r.w[1 - P], slice_rsrc(a.pwords, sl2), c, 0);
+    pk_clear(c, nxt, dw);
+    pk_expand<E>(c, r.w[P], sl1, nxt);
     sl1 = sl2;
     sl2 = s3 < c.end ? s3m : make_uint2(0u, 0u);
     lds_barrier();  // the next slice's image complete
@@ -329,36 +333,31 @@ __device__ __forceinline__ void pk_dround(const PkCtx &c, uint32_t (&w)[2][E], i
 
 // A decode wavefront (dw = 0 .. kPkDecodeWaves-1) over slices s .. end-1
 // (loads past a slice's end return zeros and fetch nothing; slices past the
-// matrix read as empty).
+// matrix read as empty).  Slice i of the range lies in image i % 2.
 template <int E>
 __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
     const DenseArgs &a = *c.a;
-    uint32_t w[2][E];
-    pk_fill(c, c.smem, c.nimg * c.img_bytes / 16u, (uint32_t)(dw * 64 + c.lane));
+    PkDRegs<E> r;
+    pk_clear(c, c.smem, dw);  // image 0's rows of this wavefront hold the background
     const uint2 s0m = pk_slice_meta(c, s);
     const uint2 s1m = pk_slice_meta(c, s + 1), s2m = pk_slice_meta(c, s + 2);
     uint2 sl1 = s + 1 < c.end ? s1m : make_uint2(0u, 0u);
     uint2 sl2 = s + 2 < c.end ? s2m : make_uint2(0u, 0u);
-    load_group<E>(w[0], slice_rsrc(a.pwords, sl1), dw * E);
-    lds_barrier();  // the images filled
-    pk_decode_groups<E>(c, s0m, dw, kPkDecodeWaves, c.smem);
-    lds_barrier();  // the first slice decoded
-    // slice i of the range lies in image i % nimg: round i decodes slice i + 1
-    // (image nx) and, with three images, fills slice i + 2's (image nf)
-    uint32_t nx = 1u % c.nimg, nf = 2u % c.nimg;
-    auto advance = [&]() {
-        nx = nx + 1 == c.nimg ? 0u : nx + 1;
-        nf = nf + 1 == c.nimg ? 0u : nf + 1;
-    };
-    pk_dround<E, 0>(c, w, dw, s, sl1, sl2, nx, nf);
+    {
+        uint32_t w0[E];
+        pk_load_rows<E>(w0, slice_rsrc(a.pwords, s0m), c, 0);
+        pk_load_rows<E>(r.w[0], slice_rsrc(a.pwords, sl1), c, 0);
+        lds_barrier();  // (the MFMA wavefronts' first barrier)
+        pk_expand<E>(c, w0, s0m, c.smem);
+    }
+    lds_barrier();  // the first slice expanded
+    pk_dround<E, 0>(c, r, dw, s, sl1, sl2);
     ++s;
     for (;;) {
         if (s >= c.end) return;
-        advance();
-        pk_dround<E, 1>(c, w, dw, s, sl1, sl2, nx, nf);
+        pk_dround<E, 1>(c, r, dw, s, sl1, sl2);
         if (++s >= c.end) return;
-        advance();
-        pk_dround<E, 0>(c, w, dw, s, sl1, sl2, nx, nf);
+        pk_dround<E, 0>(c, r, dw, s, sl1, sl2);
         ++s;
     }
 }
@@ -373,8 +372,9 @@ __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
 // before they are written back.  KB: column blocks the B registers hold (nkb <= KB; the chain runs over
 // all KB with zero B operands past nkb, so that it has no control flow).
 // Every wavefront passes the same barriers: two before the first round, then
-// per round one (three images: the end of the round) or two (two images: the
-// decode wavefronts' fill, then the end of the round).
+// one per round (its end: the next slice's image complete, this one's read).
+// Two images: the MFMA wavefronts multiply slice s from image s % 2 while the
+// decode wavefronts clear and refill image (s + 1) % 2.
 //
 // Image of a slice (LDS): row r's bytes at r * stride + column, stride =
 // cols_pad + 16; the bytes past cols meet zero B operands.  The A
@@ -403,9 +403,9 @@ __attribute__((amdgpu_waves_per_eu(PkShape::kWavesPerEU, PkShape::kWavesPerEU)))
     c.cols_pad = (uint32_t)a.cols_pad;
     c.stride = c.cols_pad + 16u;
     c.img_bytes = 64u * c.stride;
-    c.nimg = (uint32_t)a.nimg;
     c.fill = a.u8off ? 0x80808080u : 0u;                               // the image byte of an absent entry
-    c.rc = (uint32_t)lane * c.stride + (uint32_t)a.cols;                 // scatter: lane = row
+    c.drow = (uint32_t)((wave - a.NT) * 16 + (lane & 15));            // decode wavefronts' row
+    c.rc = c.drow * c.stride + (uint32_t)a.cols;
     c.a_base = (uint32_t)(lane & 31) * c.stride + (uint32_t)hi * 16u;  // A operand of row tile 0
     c.cb = a.pcb;
     c.meta = __builtin_amdgcn_make_buffer_rsrc((void *)a.pslices, (short)0, (int)(a.nslices * 8),

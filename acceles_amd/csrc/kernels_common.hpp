@@ -617,7 +617,7 @@ struct QTable {
 // lane mapping for the B operand (query l%32 of a 32-query tile), so every k
 // is paired identically in both.  Unsigned 8-bit values are stored as v-128
 // (absent entries as -128) and 128 * sum(query) is added back per query;
-// other widths are stored as is.  Two or three images, then kPkLdsTail bytes that the
+// other widths are stored as is.  Two images, then kPkLdsTail bytes that the
 // chain's reads of column blocks past nkb (multiplied by zero B operands)
 // may reach.
 static constexpr int kPkLdsTail = 34 * 32;
@@ -637,7 +637,6 @@ struct DenseArgs {
     const uint8_t *nvalid;       // valid rows per slice (a prefix of its lanes)
     int64_t nslices;
     int cols, cols_pad, nkb, col_bits, val_bits, is_signed, u8off;
-    int nimg;                    // pk_scores: slice images in LDS (2 or 3)
     const float *X;              // nq x cols FP32 queries
     int nq, NT, nqb;             // queries; 32-query tiles per block; blocks
     int8_t *qtab;                // nqb x nkb x NT x 1024 B-operand bytes
