@@ -321,13 +321,14 @@ int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq,
  * pass (host_spmv_ucsr.cpp:1084-1180); a batch here is one pass over the
  * sliced packets per block of queries, scored either on the vector ALUs
  * (ACC_BATCH_SPARSE, v_dot4) or on the matrix cores (ACC_BATCH_DENSE,
- * v_mfma_i32_32x32x32_i8: each 64-row slice is expanded from its packets
- * into an int8 image in LDS, one byte per (row, column), inside the scoring
- * kernel; nothing beyond the packets is stored).  Both give the same integer
+ * v_mfma_i32_32x32x32_i8: each 64-row slice is expanded into an int8 image
+ * in LDS, one byte per (row, column), inside the scoring kernel, from the
+ * handle's image packets -- the packets re-encoded at upload, about their
+ * size at 8 bits, acc_gpu_info_t.dense_bytes).  Both give the same integer
  * scores and the same results.  ACC_BATCH_DENSE fails with a message when
  * the matrix cores cannot score the matrix exactly (a row whose values in one
- * column sum past one byte, |score| >= 2^25, more than 1024 columns, or
- * entries wider than 18 bits); ACC_BATCH_AUTO uses them when they can and the
+ * column sum past one byte, |score| >= 2^25, or more than 1024 columns);
+ * ACC_BATCH_AUTO uses them when they can and the
  * batch has at least min_dense_batch queries (0 keeps the current value;
  * default 16). */
 #define ACC_BATCH_AUTO 0
