@@ -338,6 +338,9 @@ template <int E>
 __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
     const DenseArgs &a = *c.a;
     PkDRegs<E> r;
+    // the decode wavefronts bound each round (the MFMA wavefronts wait at its
+    // barrier): their instructions, mostly LDS stores, issue first
+    __builtin_amdgcn_s_setprio(2);
     pk_clear(c, c.smem, dw);  // image 0's rows of this wavefront hold the background
     const uint2 s0m = pk_slice_meta(c, s);
     const uint2 s1m = pk_slice_meta(c, s + 1), s2m = pk_slice_meta(c, s + 2);
