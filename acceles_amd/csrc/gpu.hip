@@ -424,8 +424,9 @@ int batch_layout(const acc_gpu_matrix *m, int nq) {
 // B registers for 8, 16, 24, 30 or 32 column blocks: the smallest that holds
 // the matrix (the multiplications run over all of them, zero past nkb); E =
 // the image packets' entry bits.
-dense_fn pk_kernel(const acc_gpu_matrix *m) {
+dense_fn pk_kernel(const acc_gpu_matrix *m, int tpw) {
     const int E = 8 + m->pcb;
+    if (tpw == 2) return m->nkb > 8 ? get_pk2_scores_16(E) : get_pk2_scores_8(E);
     if (m->nkb > 30) return get_pk_scores_32(E);
     if (m->nkb > 24) return get_pk_scores_30(E);
     if (m->nkb > 16) return get_pk_scores_24(E);
@@ -438,13 +439,19 @@ dense_fn pk_kernel(const acc_gpu_matrix *m) {
 int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, int kmax, acc_element *out,
                  int out_stride, int32_t *crow, int32_t *cscore, hipStream_t st) {
     if (!m->dense_ok) return fail(ACC_ERR_ARG, "matrix-core scores unavailable: " + m->dense_why);
-    // query blocks of at most kPkMfmaWaves tiles of 32 queries (one MFMA
-    // wavefront each), balanced; every block has kPkDecodeWaves decode
-    // wavefronts
+    // query blocks of at most kPkMfmaWaves MFMA wavefronts, balanced, with
+    // TPW tiles of 32 queries each; every block has kPkDecodeWaves decode
+    // wavefronts.  Two tiles per wavefront (NT even; a wavefront's second
+    // tile may be all padding) where both tiles' B operands fit the registers
+    // (up to 16 column blocks) and one tile each would take more than one
+    // block: fewer blocks decode each slice fewer times (256 queries at 500
+    // columns: 74 -> 59 us), while within one block one tile per wavefront
+    // spreads the MFMAs over more SIMDs (100 queries: 41 vs 53 us)
     const int tiles = (nq + 31) / 32;
-    const int nqb = (tiles + kPkMfmaWaves - 1) / kPkMfmaWaves;
-    const int NT = (tiles + nqb - 1) / nqb, QB = 32 * NT;
-    const int W = NT + kPkDecodeWaves;
+    const int tpw = m->nkb <= 16 && tiles > kPkMfmaWaves ? 2 : 1;
+    const int nqb = (tiles + kPkMfmaWaves * tpw - 1) / (kPkMfmaWaves * tpw);
+    const int NT = ((tiles + nqb - 1) / nqb + tpw - 1) / tpw * tpw, QB = 32 * NT;
+    const int W = NT / tpw + kPkDecodeWaves;
     const size_t cand = (size_t)nq * m->split * M;
     int rc = ws_grow(ws, &ws.qtab, &ws.qtab_n, (size_t)nqb * m->nkb * NT * 1024);
     if (rc == ACC_OK) rc = ws_grow(ws, &ws.qplain, &ws.qplain_n, (size_t)nq * m->cols_pad);
@@ -471,7 +478,7 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
 
     LAUNCH(m, 0, get_dense_prep(), dim3(nqb * QB), dim3(ACC_WG), 0, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
-    dense_fn fs = pk_kernel(m);
+    dense_fn fs = pk_kernel(m, tpw);
     if (!fs) return fail(ACC_ERR_ARG, "unsupported entry width");
     // two slice images (one multiplied while the other is cleared and refilled)
     const size_t img = (size_t)64 * (m->cols_pad + 16);

@@ -622,7 +622,8 @@ struct QTable {
 // may reach.
 static constexpr int kPkLdsTail = 34 * 32;
 // pk_scores workgroups: up to kPkMfmaWaves MFMA wavefronts (one 32-query tile
-// each) and kPkDecodeWaves decode wavefronts.
+// each, or two where the B operands of two tiles fit the registers: up to 16
+// column blocks) and kPkDecodeWaves decode wavefronts.
 static constexpr int kPkMfmaWaves = 4;
 static constexpr int kPkDecodeWaves = 4;
 
@@ -695,3 +696,5 @@ dense_fn get_pk_scores_16(int E);                              // k_pkmfma_b.hip
 dense_fn get_pk_scores_32(int E);                              // k_pkmfma_c.hip
 dense_fn get_pk_scores_24(int E);                              // k_pkmfma_d.hip
 dense_fn get_pk_scores_30(int E);                              // k_pkmfma_e.hip
+dense_fn get_pk2_scores_8(int E);                              // k_pkmfma_f.hip
+dense_fn get_pk2_scores_16(int E);                             // k_pkmfma_g.hip

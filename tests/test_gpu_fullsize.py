@@ -10,7 +10,8 @@
 - The serving loop over three resident matrices (1M x 500, 5M x 900,
   10M x 1024): probe queries alone, in a mix of other requests, and against
   the oracle.
-- 400,000 x 500 at 4 bits.
+- 400,000 x 500 at 4 bits; at 8 bits, batches of 100 and 256 queries on the
+  matrix cores (one and two query tiles per MFMA wavefront).
 """
 import numpy as np
 import pytest
@@ -265,6 +266,31 @@ def test_embedding_scale_4bit(acc, gpu):
         _, oi, _ = O.two_stage(ptr, idx, val, q, x, cols, g.info.group, split, ck, K)
         np.testing.assert_array_equal(a[n, :, 1], oi)
     print(f"\n[fullsize] 4-bit precision@100 {np.mean(prec):.4f}", flush=True)
+
+
+@pytest.mark.parametrize("B", [100, 256])
+def test_embedding_scale_batch(acc, gpu, B):
+    """400,000 x 500, 8-bit, a batch on the matrix cores in one launch (100
+    queries: four 32-query tiles, one per MFMA wavefront; 256: eight, two
+    per wavefront): the first and last query of every tile against the
+    oracle."""
+    rows, cols, split, ck, K = 400_000, 500, 32, 512, 100
+    print(f"\n[fullsize] 400000x500 batch of {B}", flush=True)
+    csr = acc.CSRMatrix.generate(rows, cols, deg=(40, 100), seed=SEED + 1)
+    pk = csr.pack(bits=8, split=split)
+    g = acc.GpuMatrix(csr, pk)
+    g.set_batch_mode(acc.BATCH_DENSE)
+    ptr, idx, val = csr.arrays()
+    q, _ = O.quantize(val, 8, False)
+    X = acc.generate_queries(B, cols, seed=SEED + 7)
+    out = empty(acc, B * K * 2, np.int32)
+    g.topk_lowbit_batch(dev(acc, X), np.full(B, K, np.int32), ck, out, K)
+    acc.synchronize()
+    a = out.numpy().reshape(B, K, 2)
+    for qi in sorted({q for t in range(0, B, 32) for q in (t, min(t + 31, B - 1))}):
+        _, oi, ov = O.two_stage(ptr, idx, val, q, X[qi], cols, g.info.group, split, ck, K)
+        np.testing.assert_array_equal(a[qi, :, 1], oi, err_msg=f"query {qi}")
+        np.testing.assert_array_equal(a[qi, :, 0].view(np.float32), ov, err_msg=f"query {qi}")
 
 
 # --------------------------------------------------------------------------

@@ -245,8 +245,10 @@ def test_lowbit_batch_uneven_k(acc, gpu, mode):
     (20000, 500, (40, 100), 4, 32, False, 9),    # 4-bit, one 32-query block
     (30000, 1024, (0.5, 40.0), 8, 64, False, 33),  # power-law rows, 1024 columns (32 column blocks)
     (20000, 900, (30, 90), 8, 32, False, 256),   # 900 columns, 256 queries: two 128-query blocks
-    (20000, 500, (40, 100), 8, 32, False, 256),  # 256 queries in one block of eight wavefronts
-    (12000, 300, (5, 30), 4, 8, False, 300),     # 300 queries: two blocks of five 32-query tiles, a ragged last one
+    (20000, 500, (40, 100), 8, 32, False, 256),  # 256 queries: one block, two tiles per MFMA wavefront
+    (20000, 500, (40, 100), 8, 32, False, 160),  # five tiles: three two-tile wavefronts, a padding tile
+    (20000, 200, (10, 30), 8, 8, False, 200),    # 8 column blocks, seven tiles: a ragged tile, a padding tile
+    (12000, 300, (5, 30), 4, 8, False, 300),     # 300 queries: two blocks of three two-tile wavefronts, a ragged tile, a padding tile
     (40000, 4096, (0, 300), 4, 16, False, 24),   # 4096 columns: query columns past what the kernels hold in registers
     (50000, 200, (10, 30), 8, 2, False, 20),     # partitions of 391 slices (threshold by extraction)
     (150000, 200, (10, 30), 8, 2, False, 20),    # partitions of 1172 slices (threshold by collectors)
