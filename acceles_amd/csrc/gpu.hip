@@ -26,9 +26,9 @@ using acc::fail;
 static constexpr int kMaxM = 64;     // rows kept per partition
 static constexpr int kMaxK = 960;    // Top-K limit of the FP32 path (collector cap 1024)
 static constexpr int kMaxCK = 4096;  // candidates of the low-bit path
-static constexpr size_t kWordPad = 32 * 64;  // dwords past the packets (>= widest group, 22 x 64)
+static constexpr size_t kWordPad = 32 * 64;  // dwords past the packets (>= widest group of 32 entries)
 static constexpr int kPkMaxCols = 1024;  // matrix-core scores: B operands of 32 column blocks in registers
-static constexpr int kPkMaxE = 19;       // image-packet entry widths instantiated for them
+static constexpr int kPkMinE = 18, kPkMaxE = 23;  // image-packet entry widths instantiated for them
 static constexpr size_t kPkLdsMax = 160 * 1024;  // LDS of a pk_scores workgroup at most
 
 // Per-batch device workspace of the batched path.  The handle owns one;
@@ -821,18 +821,22 @@ extern "C" int acc_gpu_upload(const acc_csr *csr, const acc_packed *pk, int devi
                 m->dense_why = "scores may reach 2^25 (too many columns for the value width)";
             else if (m->cols > kPkMaxCols)
                 m->dense_why = "more than 1024 columns";
-            else if (8 + std::max(2, acc::bits_for(m->cols + 1)) > kPkMaxE)
-                m->dense_why = "image-packet entries wider than 19 bits";
             else
                 m->dense_ok = true;
         }
         if (m->dense_ok) {
             // the matrix-core scores' own copy of the slices: image bytes and
-            // cols - column per entry, repeated columns merged (k_pkmfma.hpp)
-            m->pcb = std::max(2, acc::bits_for(m->cols + 1));
+            // their offsets in a decode wavefront's 16 image rows, repeated
+            // columns merged (k_pkmfma.hpp)
+            const uint32_t stride = (uint32_t)m->cols_pad + 16u;
+            m->pcb = acc::bits_for((int32_t)(16u * stride));
+            if (8 + m->pcb < kPkMinE || 8 + m->pcb > kPkMaxE) {
+                rc = fail(ACC_ERR_ARG, "image-packet entry width out of range");
+                goto failed;
+            }
             std::vector<uint32_t> pw;
             std::vector<std::pair<uint32_t, uint32_t>> ps;
-            TRY(acc::image_packets(pk, m->u8off, m->pcb, pw, ps));
+            TRY(acc::image_packets(pk, m->u8off, m->pcb, stride, pw, ps));
             std::vector<uint2> psl(ps.size());
             for (size_t i = 0; i < ps.size(); ++i) psl[i] = make_uint2(ps[i].first, ps[i].second);
             TRY(dalloc(m, &m->d_pwords, pw.size() + kWordPad));

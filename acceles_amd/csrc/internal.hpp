@@ -76,14 +76,19 @@ int topk_lowbit_batch_ws(acc_gpu_matrix *m, BatchWorkspace *w, const float *x_de
 // acc_permute_rows applies (out row i = in row order[i]).
 std::vector<int32_t> row_shuffle(int32_t rows, uint64_t seed);
 
-// Image packets: the slices of pk re-encoded for the matrix-core scores.  Per
-// lane (row) a bit stream of (8 + cb)-bit entries byte | ((cols - column) << 8),
-// byte = the LDS image's value (v - 128 with u8off, else v, as a byte), a
-// row's repeated columns summed into one entry (the caller checked that the
-// sums fit a byte), then zero bits; lanes interleaved per dword as in pk.
-// slices[s] = (offset in units of 64 dwords, dwords per lane).  cb must hold
-// cols (2^cb > cols).  Returns ACC_OK or an error for out-of-range sums.
-int image_packets(const acc_packed *pk, int u8off, int cb, std::vector<uint32_t> &words,
+// Image packets: the slices of pk re-encoded for the matrix-core scores, one
+// stream per slice and decode wavefront w (rows 16w .. 16w+15).  The entries
+// of those rows, byte | (offset << 8) with offset = (row - 16w) * stride +
+// column and byte = the LDS image's value (v - 128 with u8off, else v, as a
+// byte), a row's repeated columns summed into one entry (the caller checked
+// that the sums fit a byte), rows in order, are split among the 64 lanes
+// (entry j of lane l = entry l*m + j modulo the list's size, m = ceil(size /
+// 64)), n >= m entries of 8 + ob bits per lane (n a multiple of 4);
+// lanes interleaved per dword as in pk, the four streams one after another.
+// slices[s] = (offset in units of 64 dwords, n/4 of wavefront w in byte w).
+// 16 * stride must fit ob bits and stride exceed cols.  Returns ACC_OK or an
+// error for out-of-range sums.
+int image_packets(const acc_packed *pk, int u8off, int ob, uint32_t stride, std::vector<uint32_t> &words,
                   std::vector<std::pair<uint32_t, uint32_t>> &slices);
 
 // Device-memory admission: byte reservations against a fixed limit, each

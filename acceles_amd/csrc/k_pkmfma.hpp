@@ -77,19 +77,16 @@ __device__ __forceinline__ void lds_barrier() {
 // bits, and signed 8-bit whose field is the two's-complement byte), unsigned
 // 8-bit stored as v - 128 (the image background is -128 and 128 * sum(query)
 // is added back per query), and signed below 8 bits (sign-extended).
-// Entry j (0..31) of an image-packet group (E dwords of one lane = 32
-// entries of its row) stored into the row's image line: an entry is
-// byte | (c' << 8) with c' = cols - column, and its byte (already the image's
-// value) goes to rc - c' (rc = row * stride + cols).  An all-zero entry --
-// the padding after a row's last entry, or a group past the slice's end,
-// which loads as zeros -- writes 0 to column `cols`: no column of the matrix,
-// inside the row's line, and met only by zero B operands.
+// Entry j of a lane's image-packet chunk (E dwords = 32 entries) stored into
+// the decode wavefront's 16 image rows at wimg: an entry is byte | (o << 8)
+// with o = row * stride + column inside those rows, and its byte is already
+// the image's value.
 template <int E>
-__device__ __forceinline__ void pk_entry(const uint32_t (&w)[E], int j, unsigned char *img, uint32_t rc, int cb) {
+__device__ __forceinline__ void pk_entry(const uint32_t (&w)[E], int j, unsigned char *wimg, int ob) {
     const int pos = j * E, wi = pos >> 5, sh = pos & 31;
     const uint32_t hi = (wi + 1 < E) ? w[wi + 1] : 0u;
     const uint32_t bits = (sh + E <= 32) ? (w[wi] >> sh) : __builtin_amdgcn_alignbit(hi, w[wi], sh);
-    img[rc - __builtin_amdgcn_ubfe(bits, 8, cb)] = (unsigned char)bits;
+    wimg[__builtin_amdgcn_ubfe(bits, 8, ob)] = (unsigned char)bits;
 }
 
 // Wavefront roles: up to kPkMfmaWaves MFMA wavefronts (one 32-query tile
@@ -117,9 +114,9 @@ struct PkCtx {
     unsigned char *smem;
     int lane, hi, wave;
     int64_t end;  // one past this workgroup's last slice
-    uint32_t cols_pad, stride, img_bytes, fill, rc, a_base;
-    uint32_t drow;  // decode wavefronts: the lane's row (16 * dw + lane % 16)
-    int cb;                       // column bits of the image packets
+    uint32_t cols_pad, stride, img_bytes, fill, a_base;
+    uint32_t wrow;                // decode wavefronts: offset of their 16 rows in an image (16 * dw * stride)
+    int ob;                       // offset bits of the image-packet entries
     __amdgpu_buffer_rsrc_t meta;  // a.pslices
 };
 
@@ -345,31 +342,53 @@ __device__ __forceinline__ void pk_mmain2(const PkCtx &c, int qb, int t0, int64_
     pk_store_list(c, Dk1, q1, r.kk[1][0], r.kk[1][1], r.kk[1][2], r.kk[1][3], s_prev);
 }
 
-// Decode wavefront dw owns rows 16*dw .. 16*dw + 15 of every image: lane l
-// decodes row 16*dw + l%16, packet groups l/16, l/16 + 4, ... of that row's
-// stream (E dwords per group; a group past the slice's end loads as zeros).
-// Every byte of those rows is written by this wavefront only, in program
-// order, so clearing an image and expanding the next slice into it need no
-// barrier between them.
+// Decode wavefront dw owns rows 16*dw .. 16*dw + 15 of every image.  Its
+// stream of a slice holds the entries of those rows split among its lanes,
+// consecutive ones per lane, n per lane (n a multiple of 4, the same for
+// every lane; a lane's last slots repeat entries of the next lanes), so the
+// entries are spread evenly over the lanes whatever the rows' lengths.  Every byte of those rows is written
+// by this wavefront only, in program order, so clearing an image and
+// expanding the next slice into it need no barrier between them.
 template <int E>
 struct PkDRegs {
-    uint32_t w[2][E];       // the first group set of the next slice, by round parity
+    uint32_t w[2][E];       // the first chunk of the next slice, by round parity
 };
 
-// The E dwords of group set k of the lane's row (lane / 16 + 4k) of a slice.
+// This wavefront's stream of the slice with metadata m (offset in units of
+// 64 dwords; n/4 of wavefront w in byte w): a buffer over its dwords (loads
+// past its end return zeros and fetch nothing) and n.
+struct PkStream {
+    __amdgpu_buffer_rsrc_t rs;
+    int n;
+};
+
 template <int E>
-__device__ __forceinline__ void pk_load_rows(uint32_t (&w)[E], __amdgpu_buffer_rsrc_t rs, const PkCtx &c, int k) {
-    const int g = (c.lane >> 4) + 4 * k;
+__device__ __forceinline__ PkStream pk_stream(const PkCtx &c, uint2 m, int dw) {
+    const uint32_t off = __builtin_amdgcn_readfirstlane(m.x), cnt = __builtin_amdgcn_readfirstlane(m.y);
+    dw = __builtin_amdgcn_readfirstlane(dw);  // uniform: the buffer descriptor stays scalar
+    uint32_t woff = 0;
+    for (int w = 0; w < dw; ++w) woff += (((cnt >> (8 * w)) & 255u) * 4u * E + 31u) / 32u;
+    const uint32_t n = ((cnt >> (8 * dw)) & 255u) * 4u;
+    PkStream st;
+    st.rs = __builtin_amdgcn_make_buffer_rsrc((void *)(c.a->pwords + ((size_t)off + woff) * 64), (short)0,
+                                              (int)((n * E + 31u) / 32u * 256u), 0x00020000 /* 32-bit data format */);
+    st.n = (int)n;
+    return st;
+}
+
+// The E dwords of chunk k (entries 32k .. 32k+31) of the lane's stream.
+template <int E>
+__device__ __forceinline__ void pk_load_chunk(uint32_t (&w)[E], __amdgpu_buffer_rsrc_t rs, const PkCtx &c, int k) {
 #pragma unroll
     for (int i = 0; i < E; ++i)
-        w[i] = __builtin_amdgcn_raw_buffer_load_b32(rs, ((g * E + i) * 64 + (int)c.drow) * 4, 0, 2 /* nt */);
+        w[i] = __builtin_amdgcn_raw_buffer_load_b32(rs, ((k * E + i) * 64 + c.lane) * 4, 0, 2 /* nt */);
 }
 
 // This wavefront's rows of image img back to the background: 16 rows =
 // stride 16-byte chunks (fewer LDS cycles than clearing the bytes the last
 // slice wrote, one at a time, at every width up to 1024 columns).
-__device__ __forceinline__ void pk_clear(const PkCtx &c, unsigned char *img, int dw) {
-    pv4i *p4 = (pv4i *)(img + (uint32_t)dw * 16u * c.stride);
+__device__ __forceinline__ void pk_clear(const PkCtx &c, unsigned char *img) {
+    pv4i *p4 = (pv4i *)(img + c.wrow);
     const pv4i f = {(int)c.fill, (int)c.fill, (int)c.fill, (int)c.fill};
     uint32_t i = (uint32_t)c.lane;
     for (; i + 192u < c.stride; i += 256u) {
@@ -381,18 +400,29 @@ __device__ __forceinline__ void pk_clear(const PkCtx &c, unsigned char *img, int
     for (; i < c.stride; i += 64u) p4[i] = f;
 }
 
-// Slice sl (first group set in w, further sets loaded here) into image img.
+// The first m entries of a chunk (m wave-uniform, a multiple of 4).
 template <int E>
-__device__ __forceinline__ void pk_expand(const PkCtx &c, const uint32_t (&w)[E], uint2 sl, unsigned char *img) {
+__device__ __forceinline__ void pk_chunk(const uint32_t (&w)[E], int m, unsigned char *wimg, int ob) {
+    m = __builtin_amdgcn_readfirstlane(m);  // a scalar branch per four entries
 #pragma unroll
-    for (int j = 0; j < 32; ++j) pk_entry<E>(w, j, img, c.rc, c.cb);
-    if (4 * E >= (int)sl.y) return;
-    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(c.a->pwords, sl);
-    for (int k = 1; 4 * k * E < (int)sl.y; ++k) {
+    for (int q = 0; q < 8; ++q) {
+        if (4 * q >= m) return;
+#pragma unroll
+        for (int j = 4 * q; j < 4 * q + 4; ++j) pk_entry<E>(w, j, wimg, ob);
+    }
+}
+
+// A slice (stream st, first chunk in w, further chunks loaded here) into
+// image img.
+template <int E>
+__device__ __forceinline__ void pk_expand(const PkCtx &c, const uint32_t (&w)[E], const PkStream &st,
+                                          unsigned char *img) {
+    unsigned char *wimg = img + c.wrow;
+    pk_chunk<E>(w, st.n, wimg, c.ob);
+    for (int k = 1; 32 * k < st.n; ++k) {
         uint32_t x[E];
-        pk_load_rows<E>(x, rs, c, k);
-#pragma unroll
-        for (int j = 0; j < 32; ++j) pk_entry<E>(x, j, img, c.rc, c.cb);
+        pk_load_chunk<E>(x, st.rs, c, k);
+        pk_chunk<E>(x, st.n - 32 * k, wimg, c.ob);
     }
 }
 
@@ -408,45 +438,44 @@ __device__ __forceinline__ uint2 pk_slice_meta(const PkCtx &c, int64_t s) {
 }
 
 // Decode round of parity P (the MFMA wavefronts multiply slice s from image
-// P): slice s + 1 (metadata sl1, first group set in r.w[P], requested a round
+// P): slice s + 1 (metadata sl1, first chunk in r.w[P], requested a round
 // earlier) expanded into image 1 - P after this wavefront's rows of it are
-// cleared; the first group set of slice s + 2 (sl2) requested into r.w[1-P].
+// cleared; the first chunk of slice s + 2 (sl2) requested into r.w[1-P].
 // sl1, sl2 advance by one slice.
 template <int E, int P>
 __device__ __forceinline__ void pk_dround(const PkCtx &c, PkDRegs<E> &r, int dw, int64_t s, uint2 &sl1, uint2 &sl2) {
-    const DenseArgs &a = *c.a;
     unsigned char *nxt = c.smem + (1 - P) * c.img_bytes;
     const int64_t s3 = s + 3;
     const uint2 s3m = pk_slice_meta(c, s3);
-    pk_load_rows<E>(r.w[1 - P], slice_rsrc(a.pwords, sl2), c, 0);
-    pk_clear(c, nxt, dw);
-    pk_expand<E>(c, r.w[P], sl1, nxt);
+    pk_load_chunk<E>(r.w[1 - P], pk_stream<E>(c, sl2, dw).rs, c, 0);
+    pk_clear(c, nxt);
+    pk_expand<E>(c, r.w[P], pk_stream<E>(c, sl1, dw), nxt);
     sl1 = sl2;
     sl2 = s3 < c.end ? s3m : make_uint2(0u, 0u);
     lds_barrier();  // the next slice's image complete
 }
 
 // A decode wavefront (dw = 0 .. kPkDecodeWaves-1) over slices s .. end-1
-// (loads past a slice's end return zeros and fetch nothing; slices past the
-// matrix read as empty).  Slice i of the range lies in image i % 2.
+// (slices past the range read as empty).  Slice i of the range lies in image
+// i % 2.
 template <int E>
 __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
-    const DenseArgs &a = *c.a;
     PkDRegs<E> r;
     // the decode wavefronts bound each round (the MFMA wavefronts wait at its
     // barrier): their instructions, mostly LDS stores, issue first
     __builtin_amdgcn_s_setprio(2);
-    pk_clear(c, c.smem, dw);  // image 0's rows of this wavefront hold the background
+    pk_clear(c, c.smem);  // image 0's rows of this wavefront hold the background
     const uint2 s0m = pk_slice_meta(c, s);
     const uint2 s1m = pk_slice_meta(c, s + 1), s2m = pk_slice_meta(c, s + 2);
     uint2 sl1 = s + 1 < c.end ? s1m : make_uint2(0u, 0u);
     uint2 sl2 = s + 2 < c.end ? s2m : make_uint2(0u, 0u);
     {
         uint32_t w0[E];
-        pk_load_rows<E>(w0, slice_rsrc(a.pwords, s0m), c, 0);
-        pk_load_rows<E>(r.w[0], slice_rsrc(a.pwords, sl1), c, 0);
+        const PkStream st0 = pk_stream<E>(c, s0m, dw);
+        pk_load_chunk<E>(w0, st0.rs, c, 0);
+        pk_load_chunk<E>(r.w[0], pk_stream<E>(c, sl1, dw).rs, c, 0);
         lds_barrier();  // (the MFMA wavefronts' first barrier)
-        pk_expand<E>(c, w0, s0m, c.smem);
+        pk_expand<E>(c, w0, st0, c.smem);
     }
     lds_barrier();  // the first slice expanded
     pk_dround<E, 0>(c, r, dw, s, sl1, sl2);
@@ -503,10 +532,9 @@ __attribute__((amdgpu_waves_per_eu(PkShape::kWavesPerEU, PkShape::kWavesPerEU)))
     c.img_bytes = 64u * c.stride;
     c.fill = a.u8off ? 0x80808080u : 0u;                               // the image byte of an absent entry
     const int mw = a.NT / TPW;                                         // MFMA wavefronts
-    c.drow = (uint32_t)((wave - mw) * 16 + (lane & 15));               // decode wavefronts' row
-    c.rc = c.drow * c.stride + (uint32_t)a.cols;
+    c.wrow = (uint32_t)(wave - mw) * 16u * c.stride;                   // decode wavefronts' rows
     c.a_base = (uint32_t)(lane & 31) * c.stride + (uint32_t)hi * 16u;  // A operand of row tile 0
-    c.cb = a.pcb;
+    c.ob = a.pcb;
     c.meta = __builtin_amdgcn_make_buffer_rsrc((void *)a.pslices, (short)0, (int)(a.nslices * 8),
                                                0x00020000 /* 32-bit data format */);
     if (wave >= mw) pk_dmain<E>(c, wave - mw, s0);
@@ -516,15 +544,11 @@ __attribute__((amdgpu_waves_per_eu(PkShape::kWavesPerEU, PkShape::kWavesPerEU)))
 
 #define PK_CASES(KB, TPW)                       \
     switch (E) {                                \
-        case 10: return pk_scores<10, KB, TPW>; \
-        case 11: return pk_scores<11, KB, TPW>; \
-        case 12: return pk_scores<12, KB, TPW>; \
-        case 13: return pk_scores<13, KB, TPW>; \
-        case 14: return pk_scores<14, KB, TPW>; \
-        case 15: return pk_scores<15, KB, TPW>; \
-        case 16: return pk_scores<16, KB, TPW>; \
-        case 17: return pk_scores<17, KB, TPW>; \
         case 18: return pk_scores<18, KB, TPW>; \
         case 19: return pk_scores<19, KB, TPW>; \
+        case 20: return pk_scores<20, KB, TPW>; \
+        case 21: return pk_scores<21, KB, TPW>; \
+        case 22: return pk_scores<22, KB, TPW>; \
+        case 23: return pk_scores<23, KB, TPW>; \
         default: return nullptr;                \
     }

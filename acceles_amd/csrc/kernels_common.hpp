@@ -630,9 +630,11 @@ static constexpr int kPkDecodeWaves = 4;
 struct DenseArgs {
     const uint32_t *words;       // sliced packets
     const uint2 *slices;
-    const uint32_t *pwords;      // image packets (pk_scores): (byte, cols - column) entries of pcb + 8 bits
-    const uint2 *pslices;
-    int pcb;
+    const uint32_t *pwords;      // image packets (pk_scores): (byte, offset in a decode wavefront's
+                                 // 16 image rows) entries of pcb + 8 bits, one stream per wavefront
+    const uint2 *pslices;        // per slice: (offset in units of 64 dwords, entries per lane / 4 of
+                                 // decode wavefront w in byte w)
+    int pcb;                     // offset bits
     const int32_t *perm;
     const int64_t *part_slice;   // split + 1
     const uint8_t *nvalid;       // valid rows per slice (a prefix of its lanes)

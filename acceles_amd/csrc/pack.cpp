@@ -233,9 +233,10 @@ void lane_entries(const acc_packed *pk, int64_t s, int l, std::vector<std::pair<
     }
 }
 
-// The image-packet entries of one lane: repeated columns summed (column order).
+// The image entries of one lane: (image byte, column), repeated columns
+// summed, in column order; false when a sum leaves the byte.
 bool lane_image_entries(const acc_packed *pk, int64_t s, int l, int u8off, std::vector<std::pair<int32_t, int32_t>> &ent,
-                        std::vector<uint32_t> &out) {
+                        std::vector<std::pair<uint32_t, int32_t>> &out) {
     lane_entries(pk, s, l, ent);
     std::stable_sort(ent.begin(), ent.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
     out.clear();
@@ -245,62 +246,95 @@ bool lane_image_entries(const acc_packed *pk, int64_t s, int l, int u8off, std::
         while (j < ent.size() && ent[j].first == ent[i].first) sum += ent[j++].second;
         const int64_t b = u8off ? sum - 128 : sum;
         if (b < -128 || b > 127) return false;
-        out.push_back((uint32_t)(b & 0xFF) | ((uint32_t)(pk->cols - ent[i].first) << 8));
+        out.push_back({(uint32_t)(b & 0xFF), ent[i].first});
         i = j;
+    }
+    return true;
+}
+
+// The entries of decode wavefront w (rows 16w .. 16w+15) of slice s: byte |
+// (offset << 8), offset = (row - 16w) * stride + column, rows in order.
+bool wave_image_entries(const acc_packed *pk, int64_t s, int w, int u8off, uint32_t stride,
+                        std::vector<std::pair<int32_t, int32_t>> &ent, std::vector<std::pair<uint32_t, int32_t>> &row,
+                        std::vector<uint32_t> &out) {
+    out.clear();
+    for (int r = 0; r < 16; ++r) {
+        const int l = 16 * w + r;
+        if (pk->perm[(size_t)s * 64 + l] < 0) continue;
+        if (!lane_image_entries(pk, s, l, u8off, ent, row)) return false;
+        for (const auto &e : row) out.push_back(e.first | (((uint32_t)r * stride + (uint32_t)e.second) << 8));
     }
     return true;
 }
 
 }  // namespace
 
-int acc::image_packets(const acc_packed *pk, int u8off, int cb, std::vector<uint32_t> &words,
+int acc::image_packets(const acc_packed *pk, int u8off, int ob, uint32_t stride, std::vector<uint32_t> &words,
                        std::vector<std::pair<uint32_t, uint32_t>> &slices) {
     const int64_t NS = pk->nslices();
-    const int E = cb + 8;
-    if (cb < 1 || E > 32 || (int64_t(1) << cb) <= pk->cols) return fail(ACC_ERR_ARG, "image packets: column bits");
+    const int E = ob + 8;
+    if (ob < 1 || E > 32 || (int64_t(16) * stride) > (int64_t(1) << ob) || (int64_t)stride <= pk->cols)
+        return fail(ACC_ERR_ARG, "image packets: offset bits");
     slices.assign((size_t)NS, {0u, 0u});
-    std::vector<int64_t> cnt((size_t)NS, 0);
+    // per slice and wavefront: entries per lane, a multiple of 4 (<= 256)
+    std::vector<uint32_t> n4((size_t)NS * 4, 0);
     int bad = 0;
 #pragma omp parallel
     {
         std::vector<std::pair<int32_t, int32_t>> ent;
+        std::vector<std::pair<uint32_t, int32_t>> row;
         std::vector<uint32_t> out;
 #pragma omp for schedule(dynamic, 64) reduction(| : bad)
-        for (int64_t s = 0; s < NS; ++s) {
-            int64_t mx = 0;
-            for (int l = 0; l < 64; ++l) {
-                if (pk->perm[(size_t)s * 64 + l] < 0) continue;
-                if (!lane_image_entries(pk, s, l, u8off, ent, out)) bad = 1;
-                mx = std::max<int64_t>(mx, (int64_t)out.size());
+        for (int64_t s = 0; s < NS; ++s)
+            for (int w = 0; w < 4; ++w) {
+                if (!wave_image_entries(pk, s, w, u8off, stride, ent, row, out)) bad = 1;
+                const uint64_t n = (out.size() + 63) / 64;
+                n4[(size_t)s * 4 + w] = (uint32_t)((n + 3) / 4 * 4);
             }
-            cnt[(size_t)s] = mx;
-        }
     }
     if (bad) return fail(ACC_ERR_RANGE, "image packets: a row's values in one column sum past one byte");
+    auto ndw_of = [E](uint32_t n) { return (n * (uint32_t)E + 31u) / 32u; };
     uint64_t off = 0;
     for (int64_t s = 0; s < NS; ++s) {
-        const uint32_t ndw = (uint32_t)((cnt[(size_t)s] * E + 31) / 32);
-        if (off > 0xFFFFFFFFull) return fail(ACC_ERR_RANGE, "image packets exceed 2^38 bytes");
-        slices[(size_t)s] = {(uint32_t)off, ndw};
+        uint32_t ndw = 0, packed = 0;
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t n = n4[(size_t)s * 4 + w];
+            if (n > 256) return fail(ACC_ERR_RANGE, "image packets: more than 256 entries per lane");
+            ndw += ndw_of(n);
+            packed |= (n / 4) << (8 * w);
+        }
+        if (off + ndw > 0xFFFFFFFFull) return fail(ACC_ERR_RANGE, "image packets exceed 2^40 bytes");
+        slices[(size_t)s] = {(uint32_t)off, packed};
         off += ndw;
     }
     words.assign((size_t)off * 64, 0u);
 #pragma omp parallel
     {
         std::vector<std::pair<int32_t, int32_t>> ent;
+        std::vector<std::pair<uint32_t, int32_t>> row;
         std::vector<uint32_t> out;
 #pragma omp for schedule(dynamic, 64)
         for (int64_t s = 0; s < NS; ++s) {
             uint32_t *base = words.data() + (size_t)slices[(size_t)s].first * 64;
-            for (int l = 0; l < 64; ++l) {
-                if (pk->perm[(size_t)s * 64 + l] < 0) continue;
-                lane_image_entries(pk, s, l, u8off, ent, out);
-                for (size_t j = 0; j < out.size(); ++j) {
-                    const int64_t pos = (int64_t)j * E, w = pos >> 5;
-                    const int sh = (int)(pos & 31);
-                    base[(size_t)w * 64 + l] |= out[j] << sh;
-                    if (sh + E > 32) base[(size_t)(w + 1) * 64 + l] |= out[j] >> (32 - sh);
-                }
+            for (int w = 0; w < 4; ++w) {
+                const uint32_t n = n4[(size_t)s * 4 + w];
+                if (n == 0) continue;
+                wave_image_entries(pk, s, w, u8off, stride, ent, row, out);
+                // lane l takes entries l*m .. l*m + m-1 of the wavefront's
+                // list, m = ceil(size / 64) (so one store instruction spreads
+                // over the rows); its slots m .. n-1 repeat entries of the next
+                // lanes, wrapping round to the list's start (the same byte to
+                // the same place again, never twice in one instruction)
+                const size_t m = (out.size() + 63) / 64;
+                for (uint32_t j = 0; j < n; ++j)
+                    for (int l = 0; l < 64; ++l) {
+                        const size_t i = (size_t)l * m + j;
+                        const uint32_t v = out[i % out.size()];
+                        const uint32_t pos = j * (uint32_t)E, wi = pos >> 5, sh = pos & 31;
+                        base[(size_t)wi * 64 + l] |= v << sh;
+                        if (sh + E > 32) base[(size_t)(wi + 1) * 64 + l] |= v >> (32 - sh);
+                    }
+                base += (size_t)ndw_of(n) * 64;
             }
         }
     }

@@ -697,7 +697,9 @@ def main():
         names = ["prep", "scores", "select_rescore", "final"]
         t_sc = bt_ms[1]
         cols_pad = (a.cols + 31) // 32 * 32
-        bs_bytes = g.info.dense_bytes + B * a.cols * 4  # the image packets pk_scores streams, and the queries
+        # algorithmic bytes: the packed stream and the queries (the image packets
+        # pk_scores reads instead are a wider encoding of the same entries)
+        bs_bytes = pk.info.stream_bytes + B * a.cols * 4
         bs_ops = 2.0 * nslots * cols_pad * ((B + 31) // 32 * 32)
         sc_kernel = "pk_scores"
     else:

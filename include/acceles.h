@@ -323,8 +323,9 @@ int acc_gpu_topk_lowbit_batch(acc_gpu_matrix *m, const float *x_dev, int nq,
  * (ACC_BATCH_SPARSE, v_dot4) or on the matrix cores (ACC_BATCH_DENSE,
  * v_mfma_i32_32x32x32_i8: each 64-row slice is expanded into an int8 image
  * in LDS, one byte per (row, column), inside the scoring kernel, from the
- * handle's image packets -- the packets re-encoded at upload, about their
- * size at 8 bits, acc_gpu_info_t.dense_bytes).  Both give the same integer
+ * handle's image packets -- the packets re-encoded at upload with each
+ * entry's place in the image, 18 to 23 bits per entry,
+ * acc_gpu_info_t.dense_bytes).  Both give the same integer
  * scores and the same results.  ACC_BATCH_DENSE fails with a message when
  * the matrix cores cannot score the matrix exactly (a row whose values in one
  * column sum past one byte, |score| >= 2^25, or more than 1024 columns);
