@@ -423,7 +423,10 @@ int batch_layout(const acc_gpu_matrix *m, int nq) {
 
 // B registers for 8, 16, 24, 30 or 32 column blocks: the smallest that holds
 // the matrix (the multiplications run over all of them, zero past nkb); E =
-// the image packets' entry bits.
+// the image packets' entry bits.  Up to 16 column blocks the chain runs in
+// half-rounds: with one tile per MFMA wavefront its registers leave room for
+// two workgroups per CU (k_pkmfma_{a,b}), with two tiles (tpw = 2) each A
+// read feeds both (k_pkmfma_{f,g}).
 dense_fn pk_kernel(const acc_gpu_matrix *m, int tpw) {
     const int E = 8 + m->pcb;
     if (tpw == 2) return m->nkb > 8 ? get_pk2_scores_16(E) : get_pk2_scores_8(E);

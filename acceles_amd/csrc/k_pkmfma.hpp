@@ -89,13 +89,16 @@ __device__ __forceinline__ void pk_entry(const uint32_t (&w)[E], int j, unsigned
     wimg[__builtin_amdgcn_ubfe(bits, 8, ob)] = (unsigned char)bits;
 }
 
-// Wavefront roles: up to kPkMfmaWaves MFMA wavefronts (one 32-query tile
-// each, its B operands in registers) and kPkDecodeWaves decode wavefronts
-// (packets into the LDS images), two wavefronts per SIMD: while one SIMD's
-// MFMA wavefront multiplies, its decode wavefront expands the next slice.
+// Wavefront roles: up to kPkMfmaWaves MFMA wavefronts (one or two 32-query
+// tiles each, their B operands in registers) and kPkDecodeWaves decode
+// wavefronts (packets into the LDS images), one of each per SIMD: while one
+// SIMD's MFMA wavefront multiplies, its decode wavefront expands the next
+// slice.
 struct PkShape {
     static constexpr int kThreads = (kPkMfmaWaves + kPkDecodeWaves) * 64;
-    static constexpr int kWavesPerEU = 2;
+    // two wavefronts per SIMD; four (two workgroups per CU) for the
+    // one-tile half-round chain, whose registers fit 128
+    static constexpr int waves_per_eu(int mode) { return mode == 1 ? 4 : 2; }
 };
 
 // Registers of an MFMA wavefront (one object, so that the compile-time steps
@@ -247,99 +250,111 @@ __device__ __forceinline__ void pk_mmain(const PkCtx &c, int qb, int tile, int64
     }
 }
 
-// Registers of an MFMA wavefront that holds two query tiles (column blocks
-// up to 16, so that both tiles' B operands fit).  A slice is multiplied in
-// two half-rounds, one per row tile, and every A read feeds one MFMA per
-// query tile: half the LDS reads of one tile per wavefront.  Half-round rt
-// accumulates row tile rt into acc[rt] while it reduces acc[1 - rt] (the
-// other row tile, from the half-round before) into the lists kk.
-struct PkMRegs2 {
-    pv4i ar[4];        // A operands in flight (ring)
-    pv16i acc[2][2];   // [row tile][query tile]
-    int32_t kk[2][4];  // [query tile]: four-pre-key list of the slice being reduced
+// Registers of an MFMA wavefront that multiplies a slice in two
+// half-rounds, one per row tile (32 rows), for NTL query tiles: every A read
+// feeds NTL MFMAs, and only one row tile's accumulators are live per
+// half-round besides the other row tile's being reduced.  NTL = 2 halves the
+// LDS A reads per query (both tiles' B operands fit the registers at up to
+// 16 column blocks); NTL = 1 fits 128 registers at up to 16 column blocks,
+// so that two workgroups share a CU.  Half-round rt accumulates row tile rt
+// into acc[rt] while it reduces acc[1 - rt] (the other row tile, from the
+// half-round before) into the lists kk.
+template <int NTL>
+struct PkMRegsH {
+    pv4i ar[4];          // A operands in flight (ring)
+    pv16i acc[2][NTL];   // [row tile][query tile]
+    int32_t kk[NTL][4];  // [query tile]: four-pre-key list of the slice being reduced
 };
 
-template <int KB, int RT, bool LISTS, int kb>
-__device__ __forceinline__ void pk_mstep2(PkMRegs2 &r, const pv4i (&B0)[KB], const pv4i (&B1)[KB], const pv4i *A) {
+template <int KB, int NTL, int RT, bool LISTS, int kb>
+__device__ __forceinline__ void pk_hstep(PkMRegsH<NTL> &r, const pv4i (&B)[NTL][KB], const pv4i *A) {
     constexpr int RING = 4, DIST = 2;
     if constexpr (kb + DIST < KB) r.ar[(kb + DIST) % RING] = A[(kb + DIST) * 2];
     __builtin_amdgcn_sched_barrier(0);
-    r.acc[RT][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(r.ar[kb % RING], B0[kb], r.acc[RT][0], 0, 0, 0);
-    r.acc[RT][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(r.ar[kb % RING], B1[kb], r.acc[RT][1], 0, 0, 0);
-    // the other row tile's accumulators into the lists, value v at step v*KB/32
 #pragma unroll
-    for (int v = 0; v < 32; ++v)
-        if (LISTS && v * KB / 32 == kb)
+    for (int t = 0; t < NTL; ++t)
+        r.acc[RT][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(r.ar[kb % RING], B[t][kb], r.acc[RT][t], 0, 0, 0);
+    // the other row tile's accumulators into the lists, value v at step v*KB/(16*NTL)
+#pragma unroll
+    for (int v = 0; v < 16 * NTL; ++v)
+        if (LISTS && v * KB / (16 * NTL) == kb)
             pk_top4s(r.kk[v / 16][0], r.kk[v / 16][1], r.kk[v / 16][2], r.kk[v / 16][3],
                      pk_prekey(r.acc[1 - RT][v / 16][v & 15], 1 - RT, v & 15));
     __builtin_amdgcn_sched_barrier(0);
 }
 
-template <int KB, int RT, bool LISTS, int... KBS>
-__device__ __forceinline__ void pk_mchain2(std::integer_sequence<int, KBS...>, PkMRegs2 &r, const pv4i (&B0)[KB],
-                                           const pv4i (&B1)[KB], const pv4i *A) {
-    (pk_mstep2<KB, RT, LISTS, KBS>(r, B0, B1, A), ...);
+template <int KB, int NTL, int RT, bool LISTS, int... KBS>
+__device__ __forceinline__ void pk_hchain(std::integer_sequence<int, KBS...>, PkMRegsH<NTL> &r,
+                                          const pv4i (&B)[NTL][KB], const pv4i *A) {
+    (pk_hstep<KB, NTL, RT, LISTS, KBS>(r, B, A), ...);
 }
 
 // Half-round RT: row tile RT of the slice in image `img` into acc[RT].  Row
 // tile 0 completes the previous slice's lists (its row tile 1, when LISTS);
 // row tile 1 starts this slice's lists (its row tile 0).
-template <int KB, int RT, bool LISTS>
-__device__ __forceinline__ void pk_hround(const PkCtx &c, PkMRegs2 &r, const pv4i (&B0)[KB], const pv4i (&B1)[KB],
-                                          uint32_t img) {
-    r.acc[RT][0] = (pv16i){0};
-    r.acc[RT][1] = (pv16i){0};
+template <int KB, int NTL, int RT, bool LISTS>
+__device__ __forceinline__ void pk_hround(const PkCtx &c, PkMRegsH<NTL> &r, const pv4i (&B)[NTL][KB], uint32_t img) {
+#pragma unroll
+    for (int t = 0; t < NTL; ++t) r.acc[RT][t] = (pv16i){0};
     if constexpr (RT == 1) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) r.kk[0][e] = r.kk[1][e] = INT32_MIN;
+        for (int t = 0; t < NTL; ++t)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) r.kk[t][e] = INT32_MIN;
     }
     const pv4i *A = (const pv4i *)(c.smem + img * c.img_bytes + c.a_base + (uint32_t)RT * 32u * c.stride);
     r.ar[0] = A[0];
     r.ar[1] = A[2];
-    pk_mchain2<KB, RT, LISTS>(std::make_integer_sequence<int, KB>(), r, B0, B1, A);
+    pk_hchain<KB, NTL, RT, LISTS>(std::make_integer_sequence<int, KB>(), r, B, A);
 }
 
-// An MFMA wavefront with query tiles t0 and t0 + 1 of block qb over slices
-// s .. end-1 (the same barriers as pk_mmain: two before the first slice, one
-// after each slice's second half-round).
-template <int KB>
-__device__ __forceinline__ void pk_mmain2(const PkCtx &c, int qb, int t0, int64_t s) {
+// An MFMA wavefront with query tiles t0 .. t0 + NTL-1 of block qb over
+// slices s .. end-1, in half-rounds (the same barriers as pk_mmain: two
+// before the first slice, one after each slice's second half-round).
+template <int KB, int NTL>
+__device__ __forceinline__ void pk_mmainh(const PkCtx &c, int qb, int t0, int64_t s) {
     const DenseArgs &a = *c.a;
-    pv4i B0[KB], B1[KB];
+    pv4i B[NTL][KB];
     {
         const pv4i *q = (const pv4i *)a.qtab + (size_t)qb * a.nkb * a.NT * 64 + c.lane;
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-            B0[kb] = kb < a.nkb ? q[((size_t)kb * a.NT + t0) * 64] : (pv4i){0, 0, 0, 0};
-            B1[kb] = kb < a.nkb ? q[((size_t)kb * a.NT + t0 + 1) * 64] : (pv4i){0, 0, 0, 0};
-        }
+        for (int t = 0; t < NTL; ++t)
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+                B[t][kb] = kb < a.nkb ? q[((size_t)kb * a.NT + t0 + t) * 64] : (pv4i){0, 0, 0, 0};
     }
-    const int q0 = qb * a.NT * 32 + t0 * 32 + (c.lane & 31), q1 = q0 + 32;
-    const uint32_t Dk0 = ((uint32_t)(a.corr[q0] + (1 << 25)) << 6) + (uint32_t)(c.hi * 4);
-    const uint32_t Dk1 = ((uint32_t)(a.corr[q1] + (1 << 25)) << 6) + (uint32_t)(c.hi * 4);
-    PkMRegs2 r;
+    int q[NTL];
+    uint32_t Dk[NTL];
+#pragma unroll
+    for (int t = 0; t < NTL; ++t) {
+        q[t] = qb * a.NT * 32 + (t0 + t) * 32 + (c.lane & 31);
+        Dk[t] = ((uint32_t)(a.corr[q[t]] + (1 << 25)) << 6) + (uint32_t)(c.hi * 4);
+    }
+    PkMRegsH<NTL> r;
     lds_barrier();  // the images filled
     lds_barrier();  // the first slice decoded
     uint32_t img = 0;  // slice i of the range lies in image i % 2
-    pk_hround<KB, 0, false>(c, r, B0, B1, img);
-    pk_hround<KB, 1, true>(c, r, B0, B1, img);
+    pk_hround<KB, NTL, 0, false>(c, r, B, img);
+    pk_hround<KB, NTL, 1, true>(c, r, B, img);
     lds_barrier();  // this slice's image read, the next one written
     int64_t s_prev = s++;
     for (; s < c.end; s_prev = s++) {
         img ^= 1u;
-        pk_hround<KB, 0, true>(c, r, B0, B1, img);
-        pk_store_list(c, Dk0, q0, r.kk[0][0], r.kk[0][1], r.kk[0][2], r.kk[0][3], s_prev);
-        pk_store_list(c, Dk1, q1, r.kk[1][0], r.kk[1][1], r.kk[1][2], r.kk[1][3], s_prev);
-        pk_hround<KB, 1, true>(c, r, B0, B1, img);
+        pk_hround<KB, NTL, 0, true>(c, r, B, img);
+#pragma unroll
+        for (int t = 0; t < NTL; ++t)
+            pk_store_list(c, Dk[t], q[t], r.kk[t][0], r.kk[t][1], r.kk[t][2], r.kk[t][3], s_prev);
+        pk_hround<KB, NTL, 1, true>(c, r, B, img);
         lds_barrier();
     }
     // the last slice's row tile 1
 #pragma unroll
-    for (int v = 0; v < 32; ++v)
+    for (int v = 0; v < 16 * NTL; ++v)
         pk_top4s(r.kk[v / 16][0], r.kk[v / 16][1], r.kk[v / 16][2], r.kk[v / 16][3],
                  pk_prekey(r.acc[1][v / 16][v & 15], 1, v & 15));
-    pk_store_list(c, Dk0, q0, r.kk[0][0], r.kk[0][1], r.kk[0][2], r.kk[0][3], s_prev);
-    pk_store_list(c, Dk1, q1, r.kk[1][0], r.kk[1][1], r.kk[1][2], r.kk[1][3], s_prev);
+#pragma unroll
+    for (int t = 0; t < NTL; ++t)
+        pk_store_list(c, Dk[t], q[t], r.kk[t][0], r.kk[t][1], r.kk[t][2], r.kk[t][3], s_prev);
 }
 
 // Decode wavefront dw owns rows 16*dw .. 16*dw + 15 of every image.  Its
@@ -490,8 +505,11 @@ __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
 }
 
 // Scores of the 64-row slices against query block qb (QB = 32 * NT queries;
-// NT / TPW MFMA wavefronts, TPW tiles each, followed by kPkDecodeWaves decode
-// wavefronts): grid = nwx * nqb workgroups; the nqb blocks of one slice range
+// MFMA wavefronts of one tile each, or two, then kPkDecodeWaves decode
+// wavefronts).  MODE 0: one tile per MFMA wavefront, whole-slice rounds; 1:
+// one tile, half-rounds, registers for two workgroups per CU (up to 16
+// column blocks); 2: two tiles, half-rounds (up to 16 column blocks).
+// Grid = nwx * nqb workgroups; the nqb blocks of one slice range
 // wx get workgroup ids 8 apart, hence one XCD, and the second and later
 // blocks find the packets in its L2.  Workgroup wx takes the slices wx*per
 // .. wx*per + per-1 (per = ceil(nslices / nwx)): consecutive slices, so that
@@ -509,9 +527,10 @@ __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
 // rt*32 + l%32 at column kb*32 + (l/32)*16: with stride/4 = 8m + 4, any 8
 // consecutive lanes read disjoint banks.  The query table pairs each B byte
 // with the same column.
-template <int E, int KB, int TPW>
+template <int E, int KB, int MODE>
 __global__ __launch_bounds__(PkShape::kThreads)
-__attribute__((amdgpu_waves_per_eu(PkShape::kWavesPerEU, PkShape::kWavesPerEU))) void pk_scores(DenseArgs a) {
+__attribute__((amdgpu_waves_per_eu(PkShape::waves_per_eu(MODE), PkShape::waves_per_eu(MODE)))) void pk_scores(
+    DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = (int)(threadIdx.x >> 6), lane = lane_id(), hi = lane >> 5;
     const int nqb = a.nqb;
@@ -531,24 +550,25 @@ __attribute__((amdgpu_waves_per_eu(PkShape::kWavesPerEU, PkShape::kWavesPerEU)))
     c.stride = c.cols_pad + 16u;
     c.img_bytes = 64u * c.stride;
     c.fill = a.u8off ? 0x80808080u : 0u;                               // the image byte of an absent entry
-    const int mw = a.NT / TPW;                                         // MFMA wavefronts
+    const int mw = a.NT / (MODE == 2 ? 2 : 1);                         // MFMA wavefronts
     c.wrow = (uint32_t)(wave - mw) * 16u * c.stride;                   // decode wavefronts' rows
     c.a_base = (uint32_t)(lane & 31) * c.stride + (uint32_t)hi * 16u;  // A operand of row tile 0
     c.ob = a.pcb;
     c.meta = __builtin_amdgcn_make_buffer_rsrc((void *)a.pslices, (short)0, (int)(a.nslices * 8),
                                                0x00020000 /* 32-bit data format */);
     if (wave >= mw) pk_dmain<E>(c, wave - mw, s0);
-    else if constexpr (TPW == 1) pk_mmain<KB>(c, qb, wave, s0);
-    else pk_mmain2<KB>(c, qb, 2 * wave, s0);
+    else if constexpr (MODE == 0) pk_mmain<KB>(c, qb, wave, s0);
+    else if constexpr (MODE == 1) pk_mmainh<KB, 1>(c, qb, wave, s0);
+    else pk_mmainh<KB, 2>(c, qb, 2 * wave, s0);
 }
 
-#define PK_CASES(KB, TPW)                       \
+#define PK_CASES(KB, MODE)                       \
     switch (E) {                                \
-        case 18: return pk_scores<18, KB, TPW>; \
-        case 19: return pk_scores<19, KB, TPW>; \
-        case 20: return pk_scores<20, KB, TPW>; \
-        case 21: return pk_scores<21, KB, TPW>; \
-        case 22: return pk_scores<22, KB, TPW>; \
-        case 23: return pk_scores<23, KB, TPW>; \
+        case 18: return pk_scores<18, KB, MODE>; \
+        case 19: return pk_scores<19, KB, MODE>; \
+        case 20: return pk_scores<20, KB, MODE>; \
+        case 21: return pk_scores<21, KB, MODE>; \
+        case 22: return pk_scores<22, KB, MODE>; \
+        case 23: return pk_scores<23, KB, MODE>; \
         default: return nullptr;                \
     }

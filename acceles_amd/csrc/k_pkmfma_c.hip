@@ -2,4 +2,4 @@
 // (one translation unit each, for build parallelism).
 #include "k_pkmfma.hpp"
 
-dense_fn get_pk_scores_32(int E) { PK_CASES(32, 1) }
+dense_fn get_pk_scores_32(int E) { PK_CASES(32, 0) }
