@@ -426,10 +426,13 @@ int batch_layout(const acc_gpu_matrix *m, int nq) {
 // the image packets' entry bits.  Up to 16 column blocks the chain runs in
 // half-rounds: with one tile per MFMA wavefront its registers leave room for
 // two workgroups per CU (k_pkmfma_{a,b}), with two tiles (tpw = 2) each A
-// read feeds both (k_pkmfma_{f,g}).
-dense_fn pk_kernel(const acc_gpu_matrix *m, int tpw) {
+// read feeds both (k_pkmfma_{f,g}).  Above 16 column blocks, a batch of more
+// than four tiles runs eight MFMA wavefronts per workgroup, two per SIMD
+// (wide, k_pkmfma_h): one decode of each slice for up to 256 queries.
+dense_fn pk_kernel(const acc_gpu_matrix *m, int tpw, bool wide) {
     const int E = 8 + m->pcb;
     if (tpw == 2) return m->nkb > 8 ? get_pk2_scores_16(E) : get_pk2_scores_8(E);
+    if (wide) return m->nkb > 30 ? get_pk3_scores_32(E) : m->nkb > 24 ? get_pk3_scores_30(E) : get_pk3_scores_24(E);
     if (m->nkb > 30) return get_pk_scores_32(E);
     if (m->nkb > 24) return get_pk_scores_30(E);
     if (m->nkb > 16) return get_pk_scores_24(E);
@@ -452,7 +455,9 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     // spreads the MFMAs over more SIMDs (100 queries: 41 vs 53 us)
     const int tiles = (nq + 31) / 32;
     const int tpw = m->nkb <= 16 && tiles > kPkMfmaWaves ? 2 : 1;
-    const int nqb = (tiles + kPkMfmaWaves * tpw - 1) / (kPkMfmaWaves * tpw);
+    const bool wide = m->nkb > 16 && tiles > kPkMfmaWaves;  // eight one-tile MFMA wavefronts
+    const int per_block = wide ? kPkMfmaWaves3 : kPkMfmaWaves * tpw;
+    const int nqb = (tiles + per_block - 1) / per_block;
     const int NT = ((tiles + nqb - 1) / nqb + tpw - 1) / tpw * tpw, QB = 32 * NT;
     const int W = NT / tpw + kPkDecodeWaves;
     const size_t cand = (size_t)nq * m->split * M;
@@ -481,7 +486,7 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
 
     LAUNCH(m, 0, get_dense_prep(), dim3(nqb * QB), dim3(ACC_WG), 0, st, a);
     if ((rc = check_launch()) != ACC_OK) return rc;
-    dense_fn fs = pk_kernel(m, tpw);
+    dense_fn fs = pk_kernel(m, tpw, wide);
     if (!fs) return fail(ACC_ERR_ARG, "unsupported entry width");
     // two slice images (one multiplied while the other is cleared and refilled)
     const size_t img = (size_t)64 * (m->cols_pad + 16);

@@ -95,10 +95,12 @@ __device__ __forceinline__ void pk_entry(const uint32_t (&w)[E], int j, unsigned
 // SIMD's MFMA wavefront multiplies, its decode wavefront expands the next
 // slice.
 struct PkShape {
-    static constexpr int kThreads = (kPkMfmaWaves + kPkDecodeWaves) * 64;
+    // MODE 3: eight MFMA wavefronts (two per SIMD) share the decode of each
+    // slice, three wavefronts per SIMD
+    static constexpr int threads(int mode) { return (mode == 3 ? 12 : kPkMfmaWaves + kPkDecodeWaves) * 64; }
     // two wavefronts per SIMD; four (two workgroups per CU) for the
     // one-tile half-round chain, whose registers fit 128
-    static constexpr int waves_per_eu(int mode) { return mode == 1 ? 4 : 2; }
+    static constexpr int waves_per_eu(int mode) { return mode == 1 ? 4 : mode == 3 ? 3 : 2; }
 };
 
 // Registers of an MFMA wavefront (one object, so that the compile-time steps
@@ -357,6 +359,66 @@ __device__ __forceinline__ void pk_mmainh(const PkCtx &c, int qb, int t0, int64_
         pk_store_list(c, Dk[t], q[t], r.kk[t][0], r.kk[t][1], r.kk[t][2], r.kk[t][3], s_prev);
 }
 
+// The chain of MODE 3 (two MFMA wavefronts per SIMD, one query tile each):
+// half-rounds with one accumulator set, the row tile reduced to the list
+// after its chain -- the other MFMA wavefront of the SIMD multiplies
+// meanwhile -- and an A ring of three slots, so that the B operands of up to
+// 30 column blocks fit a third of the SIMD's registers.
+template <int KB, int kb>
+__device__ __forceinline__ void pk_sstep(pv4i (&ar)[3], pv16i &acc, const pv4i (&B)[KB], const pv4i *A) {
+    constexpr int RING = 3, DIST = 2;
+    if constexpr (kb + DIST < KB) ar[(kb + DIST) % RING] = A[(kb + DIST) * 2];
+    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[kb % RING], B[kb], acc, 0, 0, 0);
+}
+
+template <int KB, int... KBS>
+__device__ __forceinline__ void pk_schain(std::integer_sequence<int, KBS...>, pv4i (&ar)[3], pv16i &acc,
+                                          const pv4i (&B)[KB], const pv4i *A) {
+    (pk_sstep<KB, KBS>(ar, acc, B, A), ...);
+}
+
+// Row tile RT of the slice in image img into the list k0..k3.
+template <int KB, int RT>
+__device__ __forceinline__ void pk_shalf(const PkCtx &c, const pv4i (&B)[KB], uint32_t img, int32_t &k0, int32_t &k1,
+                                         int32_t &k2, int32_t &k3) {
+    const pv4i *A = (const pv4i *)(c.smem + img * c.img_bytes + c.a_base + (uint32_t)RT * 32u * c.stride);
+    pv4i ar[3];
+    ar[0] = A[0];
+    ar[1] = A[2];
+    pv16i acc = (pv16i){0};
+    pk_schain<KB>(std::make_integer_sequence<int, KB>(), ar, acc, B, A);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) pk_top4s(k0, k1, k2, k3, pk_prekey(acc[v], RT, v));
+}
+
+// An MFMA wavefront of MODE 3: query tile `tile` of block qb over slices
+// s .. end-1 (the same barriers as pk_mmain).
+template <int KB>
+__device__ __forceinline__ void pk_mmains(const PkCtx &c, int qb, int tile, int64_t s) {
+    const DenseArgs &a = *c.a;
+    pv4i B[KB];
+    {
+        const pv4i *q = (const pv4i *)a.qtab + (size_t)qb * a.nkb * a.NT * 64 + c.lane;
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+            B[kb] = kb < a.nkb ? q[((size_t)kb * a.NT + tile) * 64] : (pv4i){0, 0, 0, 0};
+    }
+    const int q = qb * a.NT * 32 + tile * 32 + (c.lane & 31);
+    const uint32_t Dk = ((uint32_t)(a.corr[q] + (1 << 25)) << 6) + (uint32_t)(c.hi * 4);
+    lds_barrier();  // the images filled
+    lds_barrier();  // the first slice decoded
+    uint32_t img = 0;  // slice i of the range lies in image i % 2
+    for (;;) {
+        int32_t k0 = INT32_MIN, k1 = INT32_MIN, k2 = INT32_MIN, k3 = INT32_MIN;
+        pk_shalf<KB, 0>(c, B, img, k0, k1, k2, k3);
+        pk_shalf<KB, 1>(c, B, img, k0, k1, k2, k3);
+        pk_store_list(c, Dk, q, k0, k1, k2, k3, s);
+        lds_barrier();  // this slice's image read, the next one written
+        if (++s >= c.end) return;
+        img ^= 1u;
+    }
+}
+
 // Decode wavefront dw owns rows 16*dw .. 16*dw + 15 of every image.  Its
 // stream of a slice holds the entries of those rows split among its lanes,
 // consecutive ones per lane, n per lane (n a multiple of 4, the same for
@@ -528,7 +590,7 @@ __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
 // consecutive lanes read disjoint banks.  The query table pairs each B byte
 // with the same column.
 template <int E, int KB, int MODE>
-__global__ __launch_bounds__(PkShape::kThreads)
+__global__ __launch_bounds__(PkShape::threads(MODE))
 __attribute__((amdgpu_waves_per_eu(PkShape::waves_per_eu(MODE), PkShape::waves_per_eu(MODE)))) void pk_scores(
     DenseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -559,6 +621,7 @@ __attribute__((amdgpu_waves_per_eu(PkShape::waves_per_eu(MODE), PkShape::waves_p
     if (wave >= mw) pk_dmain<E>(c, wave - mw, s0);
     else if constexpr (MODE == 0) pk_mmain<KB>(c, qb, wave, s0);
     else if constexpr (MODE == 1) pk_mmainh<KB, 1>(c, qb, wave, s0);
+    else if constexpr (MODE == 3) pk_mmains<KB>(c, qb, wave, s0);
     else pk_mmainh<KB, 2>(c, qb, 2 * wave, s0);
 }
 

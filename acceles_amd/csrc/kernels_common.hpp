@@ -623,8 +623,10 @@ struct QTable {
 static constexpr int kPkLdsTail = 34 * 32;
 // pk_scores workgroups: up to kPkMfmaWaves MFMA wavefronts (one 32-query tile
 // each, or two where the B operands of two tiles fit the registers: up to 16
-// column blocks) and kPkDecodeWaves decode wavefronts.
+// column blocks), or up to kPkMfmaWaves3 of one tile each (two per SIMD,
+// above 16 column blocks), and kPkDecodeWaves decode wavefronts.
 static constexpr int kPkMfmaWaves = 4;
+static constexpr int kPkMfmaWaves3 = 8;
 static constexpr int kPkDecodeWaves = 4;
 
 struct DenseArgs {
@@ -700,3 +702,6 @@ dense_fn get_pk_scores_24(int E);                              // k_pkmfma_d.hip
 dense_fn get_pk_scores_30(int E);                              // k_pkmfma_e.hip
 dense_fn get_pk2_scores_8(int E);                              // k_pkmfma_f.hip
 dense_fn get_pk2_scores_16(int E);                             // k_pkmfma_g.hip
+dense_fn get_pk3_scores_24(int E);                             // k_pkmfma_h.hip
+dense_fn get_pk3_scores_30(int E);
+dense_fn get_pk3_scores_32(int E);

@@ -163,6 +163,7 @@ KERNEL_SOURCES = {
     "pk_scores": ["acceles_amd/csrc/k_pkmfma.hpp", "acceles_amd/csrc/k_pkmfma_a.hip", "acceles_amd/csrc/k_pkmfma_b.hip",
                   "acceles_amd/csrc/k_pkmfma_c.hip", "acceles_amd/csrc/k_pkmfma_d.hip", "acceles_amd/csrc/k_pkmfma_e.hip",
                   "acceles_amd/csrc/k_pkmfma_f.hip", "acceles_amd/csrc/k_pkmfma_g.hip",
+                  "acceles_amd/csrc/k_pkmfma_h.hip",
                   "acceles_amd/csrc/kernels_common.hpp", "acceles_amd/csrc/device_common.hpp"],
     "lowbit_slicemax": ["acceles_amd/csrc/k_lowbit.hip", "acceles_amd/csrc/kernels_common.hpp",
                         "acceles_amd/csrc/device_common.hpp"],
