@@ -534,13 +534,14 @@ __device__ __forceinline__ void pk_dround(const PkCtx &c, PkDRegs<E> &r, int dw,
 
 // A decode wavefront (dw = 0 .. kPkDecodeWaves-1) over slices s .. end-1
 // (slices past the range read as empty).  Slice i of the range lies in image
-// i % 2.
-template <int E>
+// i % 2.  RAISE: with one MFMA wavefront per SIMD the decode wavefronts bound
+// each round (the MFMA wavefronts wait at its barrier), so their
+// instructions, mostly LDS stores, issue first; with two MFMA wavefronts per
+// SIMD these bound it, and the decode keeps the default priority.
+template <int E, bool RAISE>
 __device__ __forceinline__ void pk_dmain(const PkCtx &c, int dw, int64_t s) {
     PkDRegs<E> r;
-    // the decode wavefronts bound each round (the MFMA wavefronts wait at its
-    // barrier): their instructions, mostly LDS stores, issue first
-    __builtin_amdgcn_s_setprio(2);
+    if constexpr (RAISE) __builtin_amdgcn_s_setprio(2);
     pk_clear(c, c.smem);  // image 0's rows of this wavefront hold the background
     const uint2 s0m = pk_slice_meta(c, s);
     const uint2 s1m = pk_slice_meta(c, s + 1), s2m = pk_slice_meta(c, s + 2);
@@ -618,7 +619,7 @@ __attribute__((amdgpu_waves_per_eu(PkShape::waves_per_eu(MODE), PkShape::waves_p
     c.ob = a.pcb;
     c.meta = __builtin_amdgcn_make_buffer_rsrc((void *)a.pslices, (short)0, (int)(a.nslices * 8),
                                                0x00020000 /* 32-bit data format */);
-    if (wave >= mw) pk_dmain<E>(c, wave - mw, s0);
+    if (wave >= mw) pk_dmain<E, MODE != 3>(c, wave - mw, s0);
     else if constexpr (MODE == 0) pk_mmain<KB>(c, qb, wave, s0);
     else if constexpr (MODE == 1) pk_mmainh<KB, 1>(c, qb, wave, s0);
     else if constexpr (MODE == 3) pk_mmains<KB>(c, qb, wave, s0);
