@@ -447,12 +447,13 @@ int launch_dense(acc_gpu_matrix *m, BatchWs &ws, const float *X, int nq, int M, 
     if (!m->dense_ok) return fail(ACC_ERR_ARG, "matrix-core scores unavailable: " + m->dense_why);
     // query blocks of at most kPkMfmaWaves MFMA wavefronts, balanced, with
     // TPW tiles of 32 queries each; every block has kPkDecodeWaves decode
-    // wavefronts.  Two tiles per wavefront (NT even; a wavefront's second
-    // tile may be all padding) where both tiles' B operands fit the registers
-    // (up to 16 column blocks) and one tile each would take more than one
-    // block: fewer blocks decode each slice fewer times (256 queries at 500
-    // columns: 74 -> 59 us), while within one block one tile per wavefront
-    // spreads the MFMAs over more SIMDs (100 queries: 41 vs 53 us)
+    // wavefronts.  Where one tile per wavefront would take more than one
+    // block, fewer blocks decode each slice fewer times: up to 16 column
+    // blocks two tiles per wavefront (NT even; a wavefront's second tile may
+    // be all padding; 256 queries at 500 columns: 74 -> 59 us), above that
+    // (two tiles' B operands exceed the registers) up to kPkMfmaWaves3
+    // one-tile wavefronts, two per SIMD.  Within one block one tile per
+    // wavefront spreads the MFMAs over more SIMDs (100 queries: 41 vs 53 us)
     const int tiles = (nq + 31) / 32;
     const int tpw = m->nkb <= 16 && tiles > kPkMfmaWaves ? 2 : 1;
     const bool wide = m->nkb > 16 && tiles > kPkMfmaWaves;  // eight one-tile MFMA wavefronts
