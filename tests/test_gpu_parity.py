@@ -256,6 +256,8 @@ def test_lowbit_batch_uneven_k(acc, gpu, mode):
     (20000, 256, (10, 60), 8, 8, False, 64),     # 256 columns: image entries one column bit wider than the packets'
     (5000, 3, (1, 3), 8, 4, False, 20),          # three columns: the narrowest image entries (8 + 2 bits)
     (20000, 700, (30, 90), 8, 16, False, 50),    # 22 column blocks: the 24-block chain
+    (20000, 700, (30, 90), 8, 16, False, 300),   # 24 blocks, 300 queries: two blocks of five one-tile wavefronts, two per SIMD
+    (20000, 1024, (0, 40), 8, 32, False, 160),   # 32 blocks, five tiles in one block of one-tile wavefronts, two per SIMD
 ])
 def test_lowbit_batch_matches_oracle(acc, gpu, mode, rows, cols, deg, bits, split, signed, nq):
     kind = 1 if isinstance(deg[0], float) else 0
